@@ -1,0 +1,153 @@
+/*
+ * lenia_hip.h -- C-ABI of the MI355X Flow Lenia step engine (liblenia_hip.so).
+ *
+ * One engine owns a batch of independent worlds (any mix of supported shapes, each
+ * with its own channels, kernels and parameters) resident in HBM, and advances all of
+ * them by whole simulation steps on one HIP stream.  Plain pointers, sizes and POD
+ * structs only; every call returns an lh_status (0 = ok) and never throws.
+ *
+ * The reference has no C ABI: its step lives behind C++ entry points that operate on
+ * module globals, all returning void (lenia.h:5-28, reintegration.h:74-107,
+ * transposition.h:5-12).  Those entry points are re-provided over this ABI by the
+ * shim in lenia_ref_shim.h; the mapping of each reference phase to the calls below is
+ * given there.  Calls on one engine must be serialised by the caller.
+ */
+#ifndef LENIA_HIP_H
+#define LENIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LH_MAX_K 16    /* per world; the reference caps at MAX_KERNELS=12 (render.h:33) */
+#define LH_MAX_C 4     /* MAX_CHANNELS render.h:34 */
+#define LH_MAX_BETA 12 /* MAX_BETA render.h:35 */
+
+typedef enum {
+    LH_OK = 0,
+    LH_EINVAL = -1,       /* bad argument, shape or parameter */
+    LH_EUNSUPPORTED = -2, /* valid request the engine does not implement */
+    LH_EHIP = -3,         /* HIP runtime error (message in lh_last_error) */
+    LH_ESTATE = -4,       /* call out of order (e.g. step before commit) */
+    LH_ENOMEM = -5
+} lh_status;
+
+/* ModeType / Border / GradientType / GrowthType, render.h:125-233 */
+enum { LH_MODE_SRT = 0, LH_MODE_RT = 1, LH_MODE_ADV = 2 };
+enum { LH_BORDER_TORUS = 0, LH_BORDER_WALL = 1, LH_BORDER_NONE = 2 };
+enum { LH_GRAD_CENTRAL = 0, LH_GRAD_SOBEL = 1 };
+enum { LH_GROWTH_NONE = 0, LH_GROWTH_RT = 1, LH_GROWTH_TRANSPOSITION = 2 };
+
+/* Reference-behaviour switches; bit set = reproduce the reference, clear = physics
+ * semantics (mass-conserving transport, true gradient axes, analytic growth ...). */
+#define LH_Q_SOBEL_PERMUTE   (1u << 0)  /* Sobel axes (z,x,y): lenia.h:30-64 */
+#define LH_Q_PREALPHA_INDEX  (1u << 1)  /* alpha written at c*P+i, read at c*N+i: lenia.cpp:660,708 */
+#define LH_Q_SUM_RESET       (1u << 2)  /* S zeroed each step: lenia.cpp:912-915 */
+#define LH_Q_FACE_HALO       (1u << 3)  /* halos fill faces only: transposition.cpp */
+#define LH_Q_TRANSPORT_TORUS (1u << 4)  /* transport halos always periodic: render.cpp:538-548 */
+#define LH_Q_RT_3CH          (1u << 5)  /* SRT/RT move channels 0..2 only: reintegration.cpp:642 */
+#define LH_Q_GROWTH_LUT      (1u << 6)  /* 8-bit bell LUT: lenia.cpp:93-101, lenia.h:79-84 */
+#define LH_Q_RT_SNORM        (1u << 7)  /* sn=(1-S)/(4s^2)+S/(2s^3): lenia.cpp:730-733 */
+#define LH_Q_CLAMP01         (1u << 8)  /* clamp to [0,1] at the end of the step: lenia.cpp:978-992 */
+#define LH_Q_CD_COPY         (1u << 9)  /* non-blur central-diff copy extent: reintegration.cpp:179-186 */
+#define LH_Q_SOFTCLIP_TR     (1u << 10) /* transposition soft-clip writes BACK: lenia.cpp:935-957 */
+#define LH_Q_ALL             0x7FFu
+
+/* Simulation parameters of one world; the reference keeps these as globals
+ * (main.cpp:30-56, off_table render.h:374-386, ch[] render.h:486-492). */
+typedef struct {
+    int   mode, border, sobel_border, gradient, growth_type;
+    int   mu_vel, adv_vel, blur;
+    float dt, sigma, dd, s_norm, theta_a, gta, sta, v_damp;
+    int   asymptotic[LH_MAX_C], soft_clip[LH_MAX_C];
+    int   osc_la, osc_flow, osc_div_flow, osc_vel;
+    float osc_la_amt, osc_flow_amt, osc_div_flow_amt;
+    uint32_t quirks;
+} lh_params;
+
+/* One growth kernel: the parameter part of struct Kernel (render.h:347-361).  The
+ * engine derives the kernel field and spectrum (init_kernel lenia.cpp:199-261). */
+typedef struct {
+    int   c0, c1, radius, beta_size;
+    float b[LH_MAX_BETA];
+    float r_a, m, s, h;
+} lh_kernel;
+
+/* World extents: x = W, y = H, z = D (the reference grid is W x H x W). */
+typedef struct { int W, H, D, C, K; } lh_shape;
+
+/* Per-world fields, unpadded, channel-major, x fastest. */
+enum {
+    LH_F_A = 0,    /* GRID interior          [C][N]    (state) */
+    LH_F_GA = 1,   /* GA moving average      [C][N]    (state) */
+    LH_F_S = 2,    /* SUM interior           [N]       (state) */
+    LH_F_V = 3,    /* V_x,V_y,V_z            [C][3][N] (state) */
+    LH_F_H = 4,    /* growth H of last step  [C][N] */
+    LH_F_B = 5,    /* BACK of last step      [C][N] */
+    LH_F_MU = 6,   /* MU (RT/SRT) or clamped displacement (ADV) [C][3][N] */
+    LH_F_USUM = 7  /* U_SUM of last step     [C][N] (needs LH_OPT_KEEP_USUM) */
+};
+
+/* Options (lh_set_option). */
+enum {
+    LH_OPT_KEEP_VELOCITY = 0, /* maintain V even where the mode never reads it (default 0) */
+    LH_OPT_KEEP_USUM = 1,     /* write U_SUM for mesh views (default 0) */
+    LH_OPT_USE_GRAPH = 2,     /* replay steps from a captured hipGraph (default 1) */
+    LH_OPT_PROFILE = 3        /* time every kernel launch with HIP events (default 0) */
+};
+
+/* ---- engine lifetime --------------------------------------------------------- */
+lh_status lh_create(int device, void **engine);
+lh_status lh_destroy(void *engine);
+lh_status lh_set_option(void *engine, int option, int value);
+
+/* Adds a world; returns its id.  Must precede lh_commit. */
+lh_status lh_add_world(void *engine, const lh_shape *shape, const lh_params *params,
+                       const lh_kernel *kernels, int *world_id);
+/* Allocates HBM for all worlds, builds kernel spectra and growth tables on the GPU. */
+lh_status lh_commit(void *engine);
+
+/* ---- state --------------------------------------------------------------------- */
+/* count is in floats and must equal the field's size. Host pointers; synchronous. */
+lh_status lh_set_field(void *engine, int world, int field, const float *src, size_t count);
+lh_status lh_get_field(void *engine, int world, int field, float *dst, size_t count);
+/* Device address of a field (valid until lh_destroy); for zero-copy consumers. */
+lh_status lh_field_device_ptr(void *engine, int world, int field, float **dptr, size_t *count);
+/* Re-derives the spectra after fields were written through lh_field_device_ptr. */
+lh_status lh_state_updated(void *engine);
+
+/* ---- stepping ------------------------------------------------------------------ */
+/* Enqueues nsteps full steps of every world on the engine's stream (asynchronous). */
+lh_status lh_step(void *engine, int nsteps);
+lh_status lh_synchronize(void *engine);
+/* The engine's hipStream_t. */
+void *lh_stream(void *engine);
+
+/* ---- reductions and read-backs ------------------------------------------------- */
+/* Total mass sum_c sum_i A, accumulated in FP64 per world, into out[n_worlds]. */
+lh_status lh_world_mass(void *engine, double *out, int n_worlds);
+/* Marching-cubes case index (mesh.cpp:173-185) of one channel of field LH_F_A,
+ * LH_F_H or LH_F_USUM with the torus face pads of transpose_mesh_borders
+ * (transposition.cpp:881-995): out[N] bytes. */
+lh_status lh_mc_case(void *engine, int world, int field, int channel, float iso,
+                     int face_only, uint8_t *out);
+
+/* ---- introspection -------------------------------------------------------------- */
+int lh_world_count(void *engine);
+/* Number of distinct kernel names with recorded launches (LH_OPT_PROFILE). */
+int lh_profile_count(void *engine);
+/* i-th kernel: name, launches and total milliseconds since the last reset. */
+lh_status lh_profile_entry(void *engine, int i, const char **name, long *launches, double *ms);
+lh_status lh_profile_reset(void *engine);
+/* Whether a shape is supported by this build. */
+int lh_shape_supported(int W, int H, int D);
+const char *lh_last_error(void);
+const char *lh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

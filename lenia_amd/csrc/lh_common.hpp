@@ -1,0 +1,75 @@
+// lh_common.hpp -- data shared by the host engine and the HIP kernels.
+//
+// Every world lives in one HBM pool; a DevWorld holds its parameters and the float
+// offsets of its fields.  Fields are unpadded, channel-major, x fastest.  Spectra are
+// half spectra [z][y][kx] of float2 with kx in [0, W/2] (W/2+1 bins).
+#pragma once
+#include <stdint.h>
+
+#define LHD_MAX_K 16
+#define LHD_MAX_C 4
+
+struct DevKernel {
+    int c0, c1;
+    float h, m, s;
+};
+
+struct DevWorld {
+    int W, H, D, C, K;
+    int mode, border, sobel_border, gradient, growth_type;
+    int mu_vel, adv_vel, blur;
+    int keep_v, keep_usum, v_live, need_f;
+    int asym_mask, soft_mask;
+    int osc_la, osc_flow, osc_div_flow, osc_vel;
+    uint32_t quirks;
+    int dd;            // transport halo depth (int)off_table.dd
+    float dt, sigma, ma, sn, clip, si, theta_a, gtam, gtas, stam, stas, damp;
+    float osc_la_amt, osc_flow_amt, osc_div_flow_amt;
+    // float offsets into the pool
+    int64_t A, A2, GA, S, V, Bk, Hg, USUM, MU, Fv, Fsp, Gsp, KH, LUT, US, SUMS, BH;
+    int64_t KST, KSP;  // staging (second pool, commit only): kernel fields and spectra
+    int nkf;           // staged fields: K kernels (+1 blur kernel)
+    DevKernel kern[LHD_MAX_K];
+};
+
+// quirk bits (same values as lenia_hip.h)
+#define LHQ_SOBEL_PERMUTE   (1u << 0)
+#define LHQ_PREALPHA_INDEX  (1u << 1)
+#define LHQ_SUM_RESET       (1u << 2)
+#define LHQ_FACE_HALO       (1u << 3)
+#define LHQ_TRANSPORT_TORUS (1u << 4)
+#define LHQ_RT_3CH          (1u << 5)
+#define LHQ_GROWTH_LUT      (1u << 6)
+#define LHQ_RT_SNORM        (1u << 7)
+#define LHQ_CLAMP01         (1u << 8)
+#define LHQ_CD_COPY         (1u << 9)
+#define LHQ_SOFTCLIP_TR     (1u << 10)
+
+// Arguments common to every phase launch of one shape class.
+struct PhaseArgs {
+    float *pool;
+    float *pool2;             // staging pool (kernel spectra at commit), may be null
+    int parity;               // current A buffer: 0 -> A, 1 -> A2
+    const DevWorld *worlds;   // all worlds
+    const int *ids;           // worlds of this shape class
+    int nworlds;              // entries in ids
+    const float2 *twW, *twH, *twD;  // twiddle tables (cos, sin) for the three lengths
+};
+
+// Which real field fwd_xy transforms.
+enum { SRC_A = 0, SRC_HBLUR = 1, SRC_KERNEL = 2 };
+
+typedef void (*launch_fn)(const PhaseArgs &, int arg, void *stream);
+
+struct ShapeOps {
+    int W, H, D;
+    launch_fn fwd_xy;       // arg: SRC_*
+    launch_fn spec_z;       // arg: 0 = growth kernels, 1 = blur
+    launch_fn inv_growth;   // growth: y-IFFT, x-c2r, bell, H, B, U_SUM
+    launch_fn inv_blur;     // blur: y-IFFT, x-c2r -> U_S, SUM_S
+    launch_fn local;        // gradients, alpha, F, mu, V
+    launch_fn transport;    // SRT / RT / ADV, oscillators, TRM
+    launch_fn kspec_z;      // forward z on kernel fields -> real octant
+};
+
+const ShapeOps *lh_shape_ops(int W, int H, int D);

@@ -1,0 +1,727 @@
+// lh_engine.hip -- host side of liblenia_hip.so: world table, HBM pool, kernel spectra,
+// step scheduling (eager or replayed from a captured hipGraph), C-ABI entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/lenia_hip.h"
+#include "lh_common.hpp"
+
+void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st);
+void lh_launch_mc(const float *f, int W, int H, int D, float iso, int face_only, unsigned char *out, void *st);
+
+namespace {
+
+thread_local std::string g_err;
+
+lh_status fail(lh_status s, const std::string &msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return fail(LH_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int64_t align64(int64_t v) { return (v + 63) & ~int64_t(63); }
+
+// ---- kernel field (init_kernel lenia.cpp:199-261) ------------------------------------
+// Each cell's value depends on |r_x|, |r_y|, |r_z| only; values are evaluated on the
+// octant, then placed and summed in the reference's loop order (r_x, r_y, r_z with r_z
+// innermost) so the fp32 normaliser matches the reference's accumulation order.
+float bell_f(float x, float m, float s) {
+    const float q = (x - m) / s;
+    const double d = (double)q;
+    return (float)std::exp(-(d * d) / 2.0);
+}
+
+void kernel_field(int W, int H, int D, const lh_kernel &k, float *out) {
+    const size_t N = (size_t)W * H * D;
+    std::memset(out, 0, sizeof(float) * N);
+    const int ow = W / 2, oh = H / 2, od = D / 2;
+    const int mx = std::max(ow, std::max(oh, od)) + 1;
+    std::vector<float> oct((size_t)mx * mx * mx, 0.0f);
+    std::vector<char> in((size_t)mx * mx * mx, 0);
+    for (int ax = 0; ax <= ow; ax++)
+        for (int ay = 0; ay <= oh; ay++)
+            for (int az = 0; az <= od; az++) {
+                const float x = (float)ax, y = (float)ay, z = (float)az;
+                float dist = (float)std::sqrt((double)(x * x + y * y + z * z));
+                dist = dist / (float)k.radius * (float)k.beta_size / k.r_a;
+                if (dist < (float)k.beta_size) {
+                    const int fl = (int)std::floor((double)dist);
+                    const int index = fl < k.beta_size - 1 ? fl : k.beta_size - 1;
+                    const float fr = (float)std::fmod((double)dist, 1.0);
+                    const size_t o = ((size_t)ax * mx + ay) * mx + az;
+                    oct[o] = k.b[index] * bell_f(fr, 0.5f, 0.15f);
+                    in[o] = 1;
+                }
+            }
+    float k_cells = 0.0f;
+    for (int rx = -ow; rx < ow; rx++)
+        for (int ry = -oh; ry < oh; ry++)
+            for (int rz = -od; rz < od; rz++) {
+                const size_t o = ((size_t)std::abs(rx) * mx + std::abs(ry)) * mx + std::abs(rz);
+                if (!in[o]) continue;
+                const int px = ((rx % W) + W) % W, py = ((ry % H) + H) % H, pz = ((rz % D) + D) % D;
+                out[px + (size_t)py * W + (size_t)pz * W * H] = oct[o];
+                k_cells += oct[o];
+            }
+    for (size_t i = 0; i < N; i++) out[i] /= k_cells;
+}
+
+// 5^3 Gaussian blur kernel (precompute_blur_kernel reintegration.cpp:13-78).
+void blur_field(int W, int H, int D, float *out) {
+    const size_t N = (size_t)W * H * D;
+    std::memset(out, 0, sizeof(float) * N);
+    float k_cells = 0.0f;
+    const float s = 1.5f;
+    for (int rx = -2; rx <= 2; rx++)
+        for (int ry = -2; ry <= 2; ry++)
+            for (int rz = -2; rz <= 2; rz++) {
+                const float x = std::fabs((float)rx), y = std::fabs((float)ry), z = std::fabs((float)rz);
+                const float dist = (x * x + y * y + z * z);
+                const float g = std::exp(-dist / (s * s * s));
+                const int px = ((rx % W) + W) % W, py = ((ry % H) + H) % H, pz = ((rz % D) + D) % D;
+                out[px + (size_t)py * W + (size_t)pz * W * H] = g;
+                k_cells += g;
+            }
+    for (size_t i = 0; i < N; i++) out[i] /= k_cells;
+}
+
+// The 256 BELL_LUT entries one kernel reads (init_bell_lut lenia.cpp:93-101,
+// bell_lookup lenia.h:79-84).
+void bell_table(float m, float s, float *out) {
+    const int im = (int)(m * 255.0f), is = (int)(s * 255.0f);
+    for (int iu = 0; iu < 256; iu++)
+        out[iu] = bell_f((float)iu / 255.0f, (float)im / 255.0f, (float)is / 255.0f);
+}
+
+void twiddles(int n, float2 *out) {
+    for (int e = 0; e < n; e++) {
+        const double a = (2.0 * 3.14159265358979323846 * (double)e) / (double)n;
+        out[e] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+}
+
+struct HostWorld {
+    lh_shape shape;
+    lh_params p;
+    std::vector<lh_kernel> k;
+    DevWorld dev;
+    int cls;
+};
+
+struct ShapeClass {
+    int W, H, D;
+    const ShapeOps *ops;
+    std::vector<int> ids;
+    int *d_ids = nullptr;
+    int maxC = 0, maxNkf = 0;
+    bool any_blur = false;
+};
+
+struct Prof {
+    long launches = 0;
+    double ms = 0.0;
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<HostWorld> worlds;
+    std::vector<ShapeClass> classes;
+    bool committed = false;
+    float *pool = nullptr;
+    size_t pool_floats = 0;
+    DevWorld *d_worlds = nullptr;
+    float2 *d_tw = nullptr;
+    std::map<int, size_t> tw_off;
+    double *d_mass = nullptr;
+    unsigned char *d_mc = nullptr;
+    size_t mc_cap = 0;
+    int parity = 0;
+    bool spectra_dirty = true;
+    int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::vector<std::string> prof_names;
+    std::map<std::string, Prof> prof;
+};
+
+PhaseArgs args_for(Engine &e, ShapeClass &c, int parity, float *pool2 = nullptr) {
+    PhaseArgs a;
+    a.pool = e.pool;
+    a.pool2 = pool2;
+    a.parity = parity;
+    a.worlds = e.d_worlds;
+    a.ids = c.d_ids;
+    a.nworlds = (int)c.ids.size();
+    a.twW = e.d_tw + e.tw_off[c.W];
+    a.twH = e.d_tw + e.tw_off[c.H];
+    a.twD = e.d_tw + e.tw_off[c.D];
+    return a;
+}
+
+// Launches one phase; in profile mode brackets it with events on the engine stream.
+void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg) {
+    if (!e.opt_profile) {
+        fn(a, arg, e.stream);
+        return;
+    }
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    hipEventRecord(t0, e.stream);
+    fn(a, arg, e.stream);
+    hipEventRecord(t1, e.stream);
+    hipEventSynchronize(t1);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    std::string key = std::string(name) + "<" + std::to_string(a.nworlds) + "x" + std::to_string(e.classes.empty() ? 0 : 0) + ">";
+    key = name;
+    auto it = e.prof.find(key);
+    if (it == e.prof.end()) {
+        e.prof_names.push_back(key);
+        it = e.prof.emplace(key, Prof{}).first;
+    }
+    it->second.launches++;
+    it->second.ms += ms;
+}
+
+void enqueue_spectra(Engine &e) {
+    for (ShapeClass &c : e.classes) {
+        PhaseArgs a = args_for(e, c, e.parity);
+        run(e, "fwd_xy", c.ops->fwd_xy, a, SRC_A | (c.maxC << 8));
+    }
+}
+
+void enqueue_step(Engine &e, int parity) {
+    for (ShapeClass &c : e.classes) {
+        PhaseArgs a = args_for(e, c, parity);
+        run(e, "spec_z", c.ops->spec_z, a, 0);
+        run(e, "inv_growth", c.ops->inv_growth, a, 0);
+        if (c.any_blur) {
+            run(e, "fwd_xy_blur", c.ops->fwd_xy, a, SRC_HBLUR | ((c.maxC + 1) << 8));
+            run(e, "spec_z_blur", c.ops->spec_z, a, 1);
+            run(e, "inv_blur", c.ops->inv_blur, a, 0);
+        }
+        run(e, "local", c.ops->local, a, 0);
+        run(e, "transport", c.ops->transport, a, 0);
+        PhaseArgs an = args_for(e, c, parity ^ 1);
+        run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));
+    }
+}
+
+lh_status validate(const lh_shape &s, const lh_params &p, const lh_kernel *k) {
+    if (!lh_shape_ops(s.W, s.H, s.D))
+        return fail(LH_EUNSUPPORTED, "shape " + std::to_string(s.W) + "x" + std::to_string(s.H) + "x" +
+                                         std::to_string(s.D) + " is not built into this library");
+    if (s.C < 1 || s.C > LH_MAX_C) return fail(LH_EINVAL, "channel count must be 1..4");
+    if (s.K < 0 || s.K > LH_MAX_K) return fail(LH_EINVAL, "kernel count must be 0..16");
+    if (s.K > 0 && !k) return fail(LH_EINVAL, "kernels pointer is null");
+    for (int i = 0; i < s.K; i++) {
+        if (k[i].c0 < 0 || k[i].c0 >= s.C || k[i].c1 < 0 || k[i].c1 >= s.C)
+            return fail(LH_EINVAL, "kernel " + std::to_string(i) + ": channel out of range");
+        if (k[i].radius < 1 || k[i].beta_size < 1 || k[i].beta_size > LH_MAX_BETA || !(k[i].r_a > 0.0f))
+            return fail(LH_EINVAL, "kernel " + std::to_string(i) + ": radius/beta_size/r_a out of range");
+        if (!(k[i].s > 0.0f)) return fail(LH_EINVAL, "kernel " + std::to_string(i) + ": s must be > 0");
+    }
+    if (p.mode < 0 || p.mode > 2 || p.border < 0 || p.border > 2 || p.sobel_border < 0 || p.sobel_border > 2 ||
+        p.gradient < 0 || p.gradient > 1 || p.growth_type < 0 || p.growth_type > 2)
+        return fail(LH_EINVAL, "mode/border/gradient/growth enum out of range");
+    if (!(p.sigma > 0.0f) || !(p.dd >= 1.0f) || (int)p.dd > 4 || !(p.dd > p.sigma))
+        return fail(LH_EINVAL, "need sigma > 0, 1 <= dd <= 4, dd > sigma");
+    if (!(p.theta_a != 0.0f)) return fail(LH_EINVAL, "theta_a must be nonzero");
+    if (p.osc_la || p.osc_flow || p.osc_div_flow) return fail(LH_EUNSUPPORTED, "oscillators are not implemented");
+    return LH_OK;
+}
+
+DevWorld make_dev(const HostWorld &hw, const Engine &e) {
+    const lh_params &p = hw.p;
+    DevWorld d;
+    std::memset(&d, 0, sizeof(d));
+    d.W = hw.shape.W; d.H = hw.shape.H; d.D = hw.shape.D; d.C = hw.shape.C; d.K = hw.shape.K;
+    d.mode = p.mode; d.border = p.border; d.sobel_border = p.sobel_border; d.gradient = p.gradient;
+    d.growth_type = p.growth_type; d.mu_vel = p.mu_vel; d.adv_vel = p.adv_vel; d.blur = p.blur && p.gradient == 0;
+    d.keep_v = e.opt_keep_v;
+    d.keep_usum = e.opt_keep_usum;
+    d.v_live = p.mu_vel || (p.mode == LH_MODE_ADV && p.adv_vel) || e.opt_keep_v;
+    d.need_f = 0;
+    for (int c = 0; c < LH_MAX_C; c++) {
+        d.asym_mask |= (p.asymptotic[c] ? 1 : 0) << c;
+        d.soft_mask |= (p.soft_clip[c] ? 1 : 0) << c;
+    }
+    d.quirks = p.quirks;
+    d.dd = (int)p.dd;
+    d.dt = p.dt;
+    d.sigma = p.sigma;
+    d.ma = p.dd - p.sigma;  // init_offset_table lenia.cpp:143
+    // RT normalisation (lenia.cpp:730-733) or the exact per-source total 1/(2 sigma)^3
+    const float sigma = p.sigma;
+    if (p.quirks & LH_Q_RT_SNORM) {
+        const float s2 = 1.0f / (4.0f * sigma * sigma);
+        const float s23 = 1.0f / (2.0f * sigma * sigma * sigma);
+        d.sn = s2 * (1.0f - p.s_norm) + (p.s_norm * s23);
+    } else {
+        d.sn = 1.0f / (8.0f * sigma * sigma * sigma);
+    }
+    const float two_s = 2.0f * sigma;
+    d.clip = two_s < 1.0f ? two_s : 1.0f;
+    d.si = 0.5f + sigma;
+    d.theta_a = p.theta_a;
+    d.gtam = 1.0f - p.gta; d.gtas = p.gta;
+    d.stam = 1.0f - p.sta; d.stas = p.sta;
+    d.damp = p.v_damp;
+    for (int i = 0; i < hw.shape.K; i++) {
+        d.kern[i].c0 = hw.k[i].c0;
+        d.kern[i].c1 = hw.k[i].c1;
+        d.kern[i].h = hw.k[i].h;
+        d.kern[i].m = hw.k[i].m;
+        d.kern[i].s = hw.k[i].s;
+    }
+    d.nkf = hw.shape.K + (d.blur ? 1 : 0);
+    return d;
+}
+
+// Field layout helpers.
+size_t field_count(const HostWorld &w, int field) {
+    const size_t N = (size_t)w.shape.W * w.shape.H * w.shape.D, C = w.shape.C;
+    switch (field) {
+    case LH_F_A: case LH_F_GA: case LH_F_H: case LH_F_B: case LH_F_USUM: return C * N;
+    case LH_F_S: return N;
+    case LH_F_V: case LH_F_MU: return 3 * C * N;
+    }
+    return 0;
+}
+
+int64_t field_off(const Engine &e, const HostWorld &w, int field) {
+    const DevWorld &d = w.dev;
+    switch (field) {
+    case LH_F_A: return e.parity ? d.A2 : d.A;
+    case LH_F_GA: return d.GA;
+    case LH_F_S: return d.S;
+    case LH_F_V: return d.V;
+    case LH_F_H: return d.Hg;
+    case LH_F_B: return d.Bk;
+    case LH_F_MU: return d.MU;
+    case LH_F_USUM: return d.USUM;
+    }
+    return -1;
+}
+
+lh_status build_kernel_spectra(Engine &e) {
+    // Kernel fields are built on the host in chunks of worlds, staged in HBM, and
+    // transformed by the same fwd_xy / z kernels as the state (octant of the real part).
+    for (ShapeClass &c : e.classes) {
+        const size_t N = (size_t)c.W * c.H * c.D, Nh = (size_t)(c.W / 2 + 1) * c.H * c.D;
+        const int nkf = c.maxNkf;
+        if (nkf == 0) continue;
+        const size_t per_world = align64((int64_t)(nkf * N)) + align64((int64_t)(nkf * Nh * 2));
+        const size_t chunk = std::max<size_t>(1, std::min<size_t>(c.ids.size(), (size_t)(2ull << 30) / (per_world * 4)));
+        float *stage = nullptr;
+        HIPCHK(hipMalloc(&stage, chunk * per_world * sizeof(float)));
+        std::vector<float> host(chunk * nkf * N);
+        int *d_chunk_ids = nullptr;
+        HIPCHK(hipMalloc(&d_chunk_ids, chunk * sizeof(int)));
+        for (size_t b = 0; b < c.ids.size(); b += chunk) {
+            const size_t n = std::min(chunk, c.ids.size() - b);
+            // host fields, one thread per world slice
+            const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nth; t++)
+                th.emplace_back([&, t]() {
+                    for (size_t j = t; j < n; j += nth) {
+                        HostWorld &hw = e.worlds[c.ids[b + j]];
+                        float *dst = host.data() + j * nkf * N;
+                        for (int k = 0; k < hw.shape.K; k++) kernel_field(c.W, c.H, c.D, hw.k[k], dst + k * N);
+                        if (hw.dev.blur) blur_field(c.W, c.H, c.D, dst + hw.shape.K * N);
+                    }
+                });
+            for (auto &t : th) t.join();
+            for (size_t j = 0; j < n; j++) {
+                HostWorld &hw = e.worlds[c.ids[b + j]];
+                hw.dev.KST = (int64_t)(j * per_world);
+                hw.dev.KSP = hw.dev.KST + align64((int64_t)(nkf * N));
+                HIPCHK(hipMemcpyAsync(stage + hw.dev.KST, host.data() + j * nkf * N, sizeof(float) * hw.dev.nkf * N,
+                                      hipMemcpyHostToDevice, e.stream));
+                HIPCHK(hipMemcpyAsync(e.d_worlds + c.ids[b + j], &hw.dev, sizeof(DevWorld), hipMemcpyHostToDevice,
+                                      e.stream));
+            }
+            HIPCHK(hipMemcpyAsync(d_chunk_ids, c.ids.data() + b, n * sizeof(int), hipMemcpyHostToDevice, e.stream));
+            PhaseArgs a = args_for(e, c, e.parity, stage);
+            a.ids = d_chunk_ids;
+            a.nworlds = (int)n;
+            c.ops->fwd_xy(a, SRC_KERNEL | (nkf << 8), e.stream);
+            c.ops->kspec_z(a, 0, e.stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(e.stream));
+        }
+        HIPCHK(hipFree(d_chunk_ids));
+        HIPCHK(hipFree(stage));
+    }
+    // growth tables
+    std::vector<float> lut;
+    for (HostWorld &hw : e.worlds) {
+        if (hw.shape.K == 0) continue;
+        lut.assign((size_t)hw.shape.K * 256, 0.0f);
+        for (int k = 0; k < hw.shape.K; k++) bell_table(hw.k[k].m, hw.k[k].s, lut.data() + k * 256);
+        HIPCHK(hipMemcpyAsync(e.pool + hw.dev.LUT, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice,
+                              e.stream));
+        HIPCHK(hipStreamSynchronize(e.stream));
+    }
+    return LH_OK;
+}
+
+void destroy_graphs(Engine &e) {
+    for (int i = 0; i < 2; i++)
+        if (e.graph[i]) {
+            hipGraphExecDestroy(e.graph[i]);
+            e.graph[i] = nullptr;
+        }
+}
+
+lh_status capture(Engine &e, int parity) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(e, parity);
+    enqueue_step(e, parity ^ 1);
+    HIPCHK(hipStreamEndCapture(e.stream, &g));
+    HIPCHK(hipGraphInstantiate(&e.graph[parity], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return LH_OK;
+}
+
+Engine *as_engine(void *p) { return static_cast<Engine *>(p); }
+
+}  // namespace
+
+extern "C" {
+
+lh_status lh_create(int device, void **out) {
+    if (!out) return fail(LH_EINVAL, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LH_EINVAL, "no such device");
+    HIPCHK(hipSetDevice(device));
+    Engine *e = new Engine();
+    e->device = device;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return fail(LH_EHIP, "hipStreamCreate failed");
+    }
+    *out = e;
+    return LH_OK;
+}
+
+lh_status lh_destroy(void *p) {
+    Engine *e = as_engine(p);
+    if (!e) return LH_OK;
+    hipSetDevice(e->device);
+    hipStreamSynchronize(e->stream);
+    destroy_graphs(*e);
+    for (ShapeClass &c : e->classes) hipFree(c.d_ids);
+    hipFree(e->pool);
+    hipFree(e->d_worlds);
+    hipFree(e->d_tw);
+    hipFree(e->d_mass);
+    hipFree(e->d_mc);
+    hipStreamDestroy(e->stream);
+    delete e;
+    return LH_OK;
+}
+
+lh_status lh_set_option(void *p, int option, int value) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    switch (option) {
+    case LH_OPT_KEEP_VELOCITY:
+    case LH_OPT_KEEP_USUM:
+        if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
+        (option == LH_OPT_KEEP_VELOCITY ? e->opt_keep_v : e->opt_keep_usum) = value != 0;
+        return LH_OK;
+    case LH_OPT_USE_GRAPH: e->opt_graph = value != 0; return LH_OK;
+    case LH_OPT_PROFILE: e->opt_profile = value != 0; return LH_OK;
+    }
+    return fail(LH_EINVAL, "unknown option");
+}
+
+lh_status lh_add_world(void *p, const lh_shape *shape, const lh_params *params, const lh_kernel *kernels,
+                       int *world_id) {
+    Engine *e = as_engine(p);
+    if (!e || !shape || !params) return fail(LH_EINVAL, "null argument");
+    if (e->committed) return fail(LH_ESTATE, "worlds must be added before lh_commit");
+    lh_status s = validate(*shape, *params, kernels);
+    if (s != LH_OK) return s;
+    HostWorld hw;
+    hw.shape = *shape;
+    hw.p = *params;
+    hw.k.assign(kernels, kernels + shape->K);
+    hw.cls = -1;
+    e->worlds.push_back(hw);
+    if (world_id) *world_id = (int)e->worlds.size() - 1;
+    return LH_OK;
+}
+
+lh_status lh_commit(void *p) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    if (e->committed) return fail(LH_ESTATE, "already committed");
+    if (e->worlds.empty()) return fail(LH_ESTATE, "no worlds");
+    HIPCHK(hipSetDevice(e->device));
+    // shape classes
+    for (int i = 0; i < (int)e->worlds.size(); i++) {
+        HostWorld &hw = e->worlds[i];
+        int ci = -1;
+        for (int j = 0; j < (int)e->classes.size(); j++)
+            if (e->classes[j].W == hw.shape.W && e->classes[j].H == hw.shape.H && e->classes[j].D == hw.shape.D) ci = j;
+        if (ci < 0) {
+            ShapeClass c;
+            c.W = hw.shape.W; c.H = hw.shape.H; c.D = hw.shape.D;
+            c.ops = lh_shape_ops(c.W, c.H, c.D);
+            e->classes.push_back(c);
+            ci = (int)e->classes.size() - 1;
+        }
+        hw.cls = ci;
+        e->classes[ci].ids.push_back(i);
+    }
+    // pool layout
+    int64_t off = 0;
+    auto take = [&](int64_t n) { const int64_t o = off; off += align64(std::max<int64_t>(n, 1)); return o; };
+    for (HostWorld &hw : e->worlds) {
+        DevWorld d = make_dev(hw, *e);
+        const int64_t N = (int64_t)d.W * d.H * d.D, Nh = (int64_t)(d.W / 2 + 1) * d.H * d.D;
+        const int64_t Noct = (int64_t)(d.W / 2 + 1) * (d.H / 2 + 1) * (d.D / 2 + 1);
+        const int C = d.C, K = d.K;
+        d.A = take(C * N);
+        d.A2 = take(C * N);
+        d.GA = take(C * N);
+        d.S = take(N);
+        d.V = take(3 * C * N);
+        d.Bk = take(C * N);
+        d.Hg = take(C * N);
+        d.USUM = d.keep_usum ? take(C * N) : d.Hg;
+        d.MU = take(3 * C * N);
+        d.Fv = d.need_f ? take(3 * C * N) : d.MU;
+        d.Fsp = take((C + (d.blur ? 1 : 0)) * Nh * 2);
+        d.Gsp = take(std::max<int64_t>(K, d.blur ? C + 1 : 0) * Nh * 2);
+        d.KH = take(K * Noct);
+        d.LUT = take(K * 256);
+        d.US = d.blur ? take(C * N) : d.Hg;
+        d.SUMS = d.blur ? take(N) : d.S;
+        d.BH = d.blur ? take(Noct) : d.KH;
+        hw.dev = d;
+        ShapeClass &c = e->classes[hw.cls];
+        c.maxC = std::max(c.maxC, C);
+        c.maxNkf = std::max(c.maxNkf, d.nkf);
+        c.any_blur |= d.blur != 0;
+    }
+    e->pool_floats = (size_t)off;
+    HIPCHK(hipMalloc(&e->pool, e->pool_floats * sizeof(float)));
+    HIPCHK(hipMemsetAsync(e->pool, 0, e->pool_floats * sizeof(float), e->stream));
+    // twiddle tables
+    std::vector<float2> tw;
+    for (ShapeClass &c : e->classes)
+        for (int n : {c.W, c.H, c.D})
+            if (!e->tw_off.count(n)) {
+                e->tw_off[n] = tw.size();
+                tw.resize(tw.size() + n);
+                twiddles(n, tw.data() + e->tw_off[n]);
+            }
+    HIPCHK(hipMalloc(&e->d_tw, tw.size() * sizeof(float2)));
+    HIPCHK(hipMemcpy(e->d_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    std::vector<DevWorld> devs;
+    for (HostWorld &hw : e->worlds) devs.push_back(hw.dev);
+    HIPCHK(hipMalloc(&e->d_worlds, devs.size() * sizeof(DevWorld)));
+    HIPCHK(hipMemcpy(e->d_worlds, devs.data(), devs.size() * sizeof(DevWorld), hipMemcpyHostToDevice));
+    for (ShapeClass &c : e->classes) {
+        HIPCHK(hipMalloc(&c.d_ids, c.ids.size() * sizeof(int)));
+        HIPCHK(hipMemcpy(c.d_ids, c.ids.data(), c.ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&e->d_mass, e->worlds.size() * sizeof(double)));
+    lh_status s = build_kernel_spectra(*e);
+    if (s != LH_OK) return s;
+    e->committed = true;
+    e->spectra_dirty = true;
+    return LH_OK;
+}
+
+lh_status lh_set_field(void *p, int world, int field, const float *src, size_t count) {
+    Engine *e = as_engine(p);
+    if (!e || !src) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    HostWorld &hw = e->worlds[world];
+    if (field != LH_F_A && field != LH_F_GA && field != LH_F_S && field != LH_F_V)
+        return fail(LH_EINVAL, "only state fields (A, GA, S, V) can be set");
+    if (count != field_count(hw, field)) return fail(LH_EINVAL, "count does not match the field size");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(e->pool + field_off(*e, hw, field), src, count * sizeof(float), hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (field == LH_F_A) e->spectra_dirty = true;
+    return LH_OK;
+}
+
+lh_status lh_get_field(void *p, int world, int field, float *dst, size_t count) {
+    Engine *e = as_engine(p);
+    if (!e || !dst) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    HostWorld &hw = e->worlds[world];
+    if (field < LH_F_A || field > LH_F_USUM) return fail(LH_EINVAL, "bad field");
+    if (field == LH_F_USUM && !hw.dev.keep_usum) return fail(LH_ESTATE, "U_SUM needs LH_OPT_KEEP_USUM");
+    if (count != field_count(hw, field)) return fail(LH_EINVAL, "count does not match the field size");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(dst, e->pool + field_off(*e, hw, field), count * sizeof(float), hipMemcpyDeviceToHost,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+lh_status lh_field_device_ptr(void *p, int world, int field, float **dptr, size_t *count) {
+    Engine *e = as_engine(p);
+    if (!e || !dptr) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    if (field < LH_F_A || field > LH_F_USUM) return fail(LH_EINVAL, "bad field");
+    HostWorld &hw = e->worlds[world];
+    *dptr = e->pool + field_off(*e, hw, field);
+    if (count) *count = field_count(hw, field);
+    return LH_OK;
+}
+
+lh_status lh_state_updated(void *p) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    e->spectra_dirty = true;
+    return LH_OK;
+}
+
+lh_status lh_step(void *p, int nsteps) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (nsteps < 0) return fail(LH_EINVAL, "nsteps < 0");
+    HIPCHK(hipSetDevice(e->device));
+    if (e->spectra_dirty) {
+        enqueue_spectra(*e);
+        HIPCHK(hipGetLastError());
+        e->spectra_dirty = false;
+    }
+    int left = nsteps;
+    if (e->opt_graph && !e->opt_profile) {
+        while (left >= 2) {
+            if (!e->graph[e->parity]) {
+                lh_status s = capture(*e, e->parity);
+                if (s != LH_OK) return s;
+            }
+            HIPCHK(hipGraphLaunch(e->graph[e->parity], e->stream));
+            left -= 2;
+        }
+    }
+    for (; left > 0; left--) {
+        enqueue_step(*e, e->parity);
+        e->parity ^= 1;
+    }
+    HIPCHK(hipGetLastError());
+    return LH_OK;
+}
+
+lh_status lh_synchronize(void *p) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+void *lh_stream(void *p) {
+    Engine *e = as_engine(p);
+    return e ? (void *)e->stream : nullptr;
+}
+
+lh_status lh_world_mass(void *p, double *out, int n) {
+    Engine *e = as_engine(p);
+    if (!e || !out) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (n != (int)e->worlds.size()) return fail(LH_EINVAL, "n_worlds does not match");
+    HIPCHK(hipSetDevice(e->device));
+    PhaseArgs a = args_for(*e, e->classes[0], e->parity);
+    lh_launch_mass(a, n, e->d_mass, e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, e->d_mass, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+lh_status lh_mc_case(void *p, int world, int field, int channel, float iso, int face_only, uint8_t *out) {
+    Engine *e = as_engine(p);
+    if (!e || !out) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    HostWorld &hw = e->worlds[world];
+    if (field != LH_F_A && field != LH_F_H && field != LH_F_USUM) return fail(LH_EINVAL, "field must be A, H or U_SUM");
+    if (field == LH_F_USUM && !hw.dev.keep_usum) return fail(LH_ESTATE, "U_SUM needs LH_OPT_KEEP_USUM");
+    if (channel < 0 || channel >= hw.shape.C) return fail(LH_EINVAL, "bad channel");
+    const size_t N = (size_t)hw.shape.W * hw.shape.H * hw.shape.D;
+    HIPCHK(hipSetDevice(e->device));
+    if (e->mc_cap < N) {
+        hipFree(e->d_mc);
+        e->d_mc = nullptr;
+        HIPCHK(hipMalloc(&e->d_mc, N));
+        e->mc_cap = N;
+    }
+    const float *f = e->pool + field_off(*e, hw, field) + (size_t)channel * N;
+    lh_launch_mc(f, hw.shape.W, hw.shape.H, hw.shape.D, iso, face_only, e->d_mc, e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, e->d_mc, N, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+int lh_world_count(void *p) {
+    Engine *e = as_engine(p);
+    return e ? (int)e->worlds.size() : 0;
+}
+
+int lh_profile_count(void *p) {
+    Engine *e = as_engine(p);
+    return e ? (int)e->prof_names.size() : 0;
+}
+
+lh_status lh_profile_entry(void *p, int i, const char **name, long *launches, double *ms) {
+    Engine *e = as_engine(p);
+    if (!e || i < 0 || i >= (int)e->prof_names.size()) return fail(LH_EINVAL, "bad profile index");
+    const std::string &k = e->prof_names[i];
+    if (name) *name = k.c_str();
+    if (launches) *launches = e->prof[k].launches;
+    if (ms) *ms = e->prof[k].ms;
+    return LH_OK;
+}
+
+lh_status lh_profile_reset(void *p) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    for (auto &kv : e->prof) kv.second = Prof{};
+    return LH_OK;
+}
+
+int lh_shape_supported(int W, int H, int D) { return lh_shape_ops(W, H, D) != nullptr; }
+
+const char *lh_last_error(void) { return g_err.c_str(); }
+
+const char *lh_version(void) { return "lenia_hip 0.1 gfx950"; }
+
+}  // extern "C"

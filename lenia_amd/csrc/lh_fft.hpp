@@ -1,0 +1,108 @@
+// lh_fft.hpp -- register-resident 1-D complex FFT for one thread (CDNA4 / gfx950).
+//
+// One thread owns a whole line of N complex values in VGPRs; every index is a
+// compile-time constant after unrolling, so there is no LDS traffic and no dynamic
+// register indexing inside a transform.  The schedule is Stockham autosort with the
+// radix plan 4,...,4,[2],odd primes ascending, twiddles W_n^e = (cos, -/+sin)(2 pi e/n)
+// read from a host-built table (double precision rounded once to fp32), odd radices
+// as a direct DFT accumulated in index order.  Built with -ffp-contract=off, so each
+// butterfly rounds exactly as written; the CPU checker restates the same schedule.
+#pragma once
+#include <hip/hip_runtime.h>
+
+constexpr int fft_nst(int n) {
+    int m = n, st = 0;
+    while (m % 4 == 0) { m /= 4; st++; }
+    if (m % 2 == 0) { m /= 2; st++; }
+    for (int q = 3; m > 1; q += 2)
+        while (m % q == 0) { m /= q; st++; }
+    return st;
+}
+
+constexpr int fft_radix(int n, int s) {
+    int m = n, st = 0;
+    while (m % 4 == 0) { if (st == s) return 4; m /= 4; st++; }
+    if (m % 2 == 0) { if (st == s) return 2; m /= 2; st++; }
+    for (int q = 3; m > 1; q += 2)
+        while (m % q == 0) { if (st == s) return q; m /= q; st++; }
+    return 1;
+}
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 w) {
+    return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ float2 twiddle(const float2 *__restrict__ tw, int e) {
+    float2 t = tw[e];
+    return INV ? t : make_float2(t.x, -t.y);
+}
+
+template <int R, int N, bool INV>
+__device__ __forceinline__ void fft_butterfly(float2 (&v)[R], const float2 *__restrict__ tw) {
+    if constexpr (R == 2) {
+        float2 a = v[0], b = v[1];
+        v[0] = c_add(a, b);
+        v[1] = c_sub(a, b);
+    } else if constexpr (R == 4) {
+        float2 a = c_add(v[0], v[2]), b = c_sub(v[0], v[2]);
+        float2 c = c_add(v[1], v[3]), d = c_sub(v[1], v[3]);
+        v[0] = c_add(a, c);
+        v[2] = c_sub(a, c);
+        float2 p = make_float2(b.x + d.y, b.y - d.x);  // b - i d
+        float2 q = make_float2(b.x - d.y, b.y + d.x);  // b + i d
+        v[1] = INV ? q : p;
+        v[3] = INV ? p : q;
+    } else {
+        float2 y[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                const int e = (j * k) % R;
+                if (e == 0) acc = c_add(acc, v[j]);
+                else acc = c_add(acc, c_mul(v[j], twiddle<INV>(tw, e * (N / R))));
+            }
+            y[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = y[k];
+    }
+}
+
+template <int N, bool INV, int S, int NS>
+__device__ __forceinline__ void fft_stages(float2 (&x)[N], const float2 *__restrict__ tw) {
+    if constexpr (S < fft_nst(N)) {
+        constexpr int R = fft_radix(N, S);
+        constexpr int NR = N / R;
+        constexpr int TS = N / (NS * R);
+        float2 y[N];
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            const int k = j % NS;
+            float2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = x[j + r * NR];
+            if (k != 0) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[r] = c_mul(v[r], twiddle<INV>(tw, r * k * TS));
+            }
+            fft_butterfly<R, N, INV>(v, tw);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) y[base + r * NS] = v[r];
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) x[i] = y[i];
+        fft_stages<N, INV, S + 1, NS * R>(x, tw);
+    }
+}
+
+// In-place unnormalised DFT of x; INV selects the +i exponent.
+template <int N, bool INV>
+__device__ __forceinline__ void fft_line(float2 (&x)[N], const float2 *__restrict__ tw) {
+    fft_stages<N, INV, 0, 1>(x, tw);
+}

@@ -1,0 +1,816 @@
+// lh_kernels.hip -- the Flow Lenia step as HIP phase kernels for gfx950.
+//
+// Each kernel is specialised on the world shape (W, H, D) and processes every world of
+// that shape in one launch (one workgroup never straddles two worlds).  A step is
+//
+//   spec_z      z-FFT of the channel half spectra, x kernel spectrum, inverse z-FFT
+//   inv_growth  inverse y-FFT, c2r along x, growth bell -> H, B (, U_SUM)   per z-plane
+//   [blur]      fwd_xy(H,S) -> spec_z(blur) -> inv_blur -> U_S, SUM_S       central diff
+//   local       gradients of S and H, alpha, F, mu, V                       per cell
+//   transport   SRT / RT gather or advection, end-of-step TRM               per cell
+//   fwd_xy      r2c along x + y-FFT of the new A per z-plane (next step's spectrum)
+//
+// Reference phases replaced: fft_step/mt_fft_step lenia.cpp:435-594, h_sum_blur
+// reintegration.cpp:89-188, transpose_*_borders transposition.cpp:6-876 (halos are
+// resolved on the fly from interior values), sobel_convolve[_h]/compute_f lenia.cpp:597-719,
+// compute_*_mu reintegration.cpp:321-602, reintegration_step lenia.cpp:727-918 with
+// srt/rt_*/adv_* reintegration.cpp:607-1410, transpose_to_row_major lenia.cpp:924-1046.
+//
+// All arithmetic is fp32 in a fixed order (built with -ffp-contract=off): results are
+// bit-identical to the CPU restatement used as the checker.
+#include <hip/hip_runtime.h>
+#include "lh_common.hpp"
+#include "lh_fft.hpp"
+
+#define PADR 4  // reference PADDING (render.h:291), used by the index-quirk emulation
+
+__device__ __forceinline__ float fmax_ref(float a, float b) { return a > b ? a : b; }  // _mm_max_ps
+__device__ __forceinline__ float fmin_ref(float a, float b) { return a < b ? a : b; }  // _mm_min_ps
+__device__ __forceinline__ float clamp_ref(float a, float lo, float hi) { return fmin_ref(fmax_ref(a, lo), hi); }
+
+template <int W, int H, int D>
+struct Shape {
+    static constexpr int N = W * H * D;
+    static constexpr int PL = W * H;
+    static constexpr int Wh = W / 2 + 1, Hh = H / 2 + 1, Dh = D / 2 + 1;
+    static constexpr int Nh = Wh * H * D;
+    static constexpr int Noct = Wh * Hh * Dh;
+    static constexpr int PW = W + 2 * PADR, PH = H + 2 * PADR, PD = D + 2 * PADR;
+    static constexpr long long P = (long long)PW * PH * PD;
+    // reference padded index PIDX of an interior-or-pad coordinate
+    __device__ static __forceinline__ long long pidx(int x, int y, int z) {
+        return (long long)(x + PADR) + (long long)(y + PADR) * PW + (long long)(z + PADR) * PW * PH;
+    }
+};
+
+// Resolves a possibly-out-of-range coordinate to the interior cell whose value a
+// reference halo holds there.  kind: 0 torus, 1 wall (replicate), 2 zero.  With
+// faces != 0 only the six faces are filled (edges/corners stay 0), as in
+// transposition.cpp.  Returns false where the halo holds 0.
+template <int W, int H, int D>
+__device__ __forceinline__ bool halo_src(int &x, int &y, int &z, int kind, int faces) {
+    const int ox = (x < 0) | (x >= W), oy = (y < 0) | (y >= H), oz = (z < 0) | (z >= D);
+    const int m = ox + oy + oz;
+    if (m == 0) return true;
+    if (kind == 2 || (faces && m >= 2)) return false;
+    if (kind == 0) {
+        x = x < 0 ? x + W : (x >= W ? x - W : x);
+        y = y < 0 ? y + H : (y >= H ? y - H : y);
+        z = z < 0 ? z + D : (z >= D ? z - D : z);
+    } else {
+        x = x < 0 ? 0 : (x >= W ? W - 1 : x);
+        y = y < 0 ? 0 : (y >= H ? H - 1 : y);
+        z = z < 0 ? 0 : (z >= D ? D - 1 : z);
+    }
+    return true;
+}
+
+__device__ __forceinline__ const float *A_cur(const PhaseArgs &a, const DevWorld &w) {
+    return a.pool + (a.parity ? w.A2 : w.A);
+}
+__device__ __forceinline__ float *A_next(const PhaseArgs &a, const DevWorld &w) {
+    return a.pool + (a.parity ? w.A : w.A2);
+}
+
+// ------------------------------------------------------------------------------------
+// fwd_xy: r2c along x (rows y, y+1 packed as one complex line) and the y-FFT, for NP
+// z-planes of one real field per workgroup.  Output: half spectrum planes [z][y][kx].
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+struct FwdCfg {
+    static constexpr int NP = (H <= 32 ? (D % 8 == 0 ? 8 : (D % 4 == 0 ? 4 : (D % 2 == 0 ? 2 : 1)))
+                                       : (D % 4 == 0 ? 4 : (D % 2 == 0 ? 2 : 1)));
+    static constexpr int THREADS = 128;
+};
+
+template <int W, int H, int D>
+__global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
+    using S = Shape<W, H, D>;
+    using Cfg = FwdCfg<W, H, D>;
+    constexpr int NP = Cfg::NP, Wh = S::Wh;
+    __shared__ float sreal[NP * W * H];
+    __shared__ float2 sspec[NP * H * Wh];
+    const int src = arg & 0xFF, nfmax = arg >> 8;
+    const int groups = D / NP;
+    const int per_world = nfmax * groups;
+    const int wi = blockIdx.x / per_world;
+    const int f = (blockIdx.x % per_world) / groups;
+    const int z0 = (blockIdx.x % groups) * NP;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    int nf;
+    const float *in;
+    float2 *out;
+    if (src == SRC_A) {
+        nf = w.C;
+        in = A_cur(a, w) + (long long)f * S::N;
+        out = reinterpret_cast<float2 *>(a.pool + w.Fsp) + (long long)f * S::Nh;
+    } else if (src == SRC_HBLUR) {
+        nf = w.C + 1;
+        in = f < w.C ? a.pool + w.Hg + (long long)f * S::N : a.pool + w.S;
+        out = reinterpret_cast<float2 *>(a.pool + w.Fsp) + (long long)f * S::Nh;
+    } else {
+        nf = w.nkf;
+        in = a.pool2 + w.KST + (long long)f * S::N;
+        out = reinterpret_cast<float2 *>(a.pool2 + w.KSP) + (long long)f * S::Nh;
+    }
+    if (f >= nf) return;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NP * W * H; i += Cfg::THREADS) sreal[i] = in[(long long)z0 * S::PL + i];
+    __syncthreads();
+    for (int t = tid; t < NP * (H / 2); t += Cfg::THREADS) {
+        const int p = t / (H / 2), y = 2 * (t % (H / 2));
+        float2 line[W];
+        const float *r0 = sreal + p * S::PL + y * W;
+#pragma unroll
+        for (int x = 0; x < W; x++) line[x] = make_float2(r0[x], r0[x + W]);
+        fft_line<W, false>(line, a.twW);
+        float2 *o0 = sspec + (p * H + y) * Wh, *o1 = o0 + Wh;
+#pragma unroll
+        for (int k = 0; k < Wh; k++) {
+            const float2 Z = line[k], P = line[(W - k) % W];
+            o0[k] = make_float2((Z.x + P.x) * 0.5f, (Z.y - P.y) * 0.5f);
+            o1[k] = make_float2((Z.y + P.y) * 0.5f, (P.x - Z.x) * 0.5f);
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < NP * Wh; t += Cfg::THREADS) {
+        const int p = t / Wh, kx = t % Wh;
+        float2 line[H];
+#pragma unroll
+        for (int y = 0; y < H; y++) line[y] = sspec[(p * H + y) * Wh + kx];
+        fft_line<H, false>(line, a.twH);
+        float2 *o = out + (long long)(z0 + p) * H * Wh + kx;
+#pragma unroll
+        for (int y = 0; y < H; y++) o[y * Wh] = line[y];
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// spec_z: per (kx, ky) column.  mode 0: for each channel used as a kernel input, z-FFT,
+// then for each kernel reading it: x real kernel octant, inverse z-FFT -> G_k.
+// mode 1 (blur): fields 0..C (H_c, S) x blur octant -> G_f.
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+__global__ __launch_bounds__(128) void k_spec_z(PhaseArgs a, int mode) {
+    using S = Shape<W, H, D>;
+    constexpr int Wh = S::Wh, Hh = S::Hh, COLS = Wh * H;
+    constexpr int BPW = (COLS + 127) / 128;
+    const int wi = blockIdx.x / BPW;
+    const int col = (blockIdx.x % BPW) * 128 + threadIdx.x;
+    if (col >= COLS) return;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int kx = col % Wh, ky = col / Wh;
+    const int kyo = ky <= H / 2 ? ky : H - ky;
+    const float2 *Fsp = reinterpret_cast<const float2 *>(a.pool + w.Fsp);
+    float2 *Gsp = reinterpret_cast<float2 *>(a.pool + w.Gsp);
+    const int nsrc = mode == 0 ? w.C : w.C + 1;
+    for (int c = 0; c < nsrc; c++) {
+        if (mode == 0) {
+            bool used = false;
+            for (int k = 0; k < w.K; k++) used |= (w.kern[k].c0 == c);
+            if (!used) continue;
+        }
+        float2 t[D];
+        const float2 *src = Fsp + (long long)c * S::Nh + ky * Wh + kx;
+#pragma unroll
+        for (int z = 0; z < D; z++) t[z] = src[(long long)z * H * Wh];
+        fft_line<D, false>(t, a.twD);
+        const int nk = mode == 0 ? w.K : 1;
+        for (int k = 0; k < nk; k++) {
+            if (mode == 0 && w.kern[k].c0 != c) continue;
+            const float *oct = mode == 0 ? a.pool + w.KH + (long long)k * S::Noct : a.pool + w.BH;
+            float2 y[D];
+#pragma unroll
+            for (int z = 0; z < D; z++) {
+                const int zo = z <= D / 2 ? z : D - z;
+                const float kv = oct[(zo * Hh + kyo) * Wh + kx];
+                y[z] = make_float2(t[z].x * kv, t[z].y * kv);
+            }
+            fft_line<D, true>(y, a.twD);
+            float2 *dst = Gsp + (long long)(mode == 0 ? k : c) * S::Nh + ky * Wh + kx;
+#pragma unroll
+            for (int z = 0; z < D; z++) dst[(long long)z * H * Wh] = y[z];
+        }
+    }
+}
+
+// kspec_z: forward z-FFT of staged kernel spectra; keeps the real part of the octant.
+template <int W, int H, int D>
+__global__ __launch_bounds__(128) void k_kspec_z(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    constexpr int Wh = S::Wh, Hh = S::Hh, Dh = S::Dh, COLS = Wh * H;
+    constexpr int BPW = (COLS + 127) / 128;
+    const int wi = blockIdx.x / BPW;
+    const int col = (blockIdx.x % BPW) * 128 + threadIdx.x;
+    if (col >= COLS) return;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int kx = col % Wh, ky = col / Wh;
+    if (ky > H / 2) return;
+    const float2 *Ksp = reinterpret_cast<const float2 *>(a.pool2 + w.KSP);
+    for (int f = 0; f < w.nkf; f++) {
+        float2 t[D];
+        const float2 *src = Ksp + (long long)f * S::Nh + ky * Wh + kx;
+#pragma unroll
+        for (int z = 0; z < D; z++) t[z] = src[(long long)z * H * Wh];
+        fft_line<D, false>(t, a.twD);
+        float *oct = f < w.K ? a.pool + w.KH + (long long)f * S::Noct : a.pool + w.BH;
+#pragma unroll
+        for (int z = 0; z < Dh; z++) oct[(z * Hh + ky) * Wh + kx] = t[z].x;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// inv_growth: per (world, z-plane).  For kernels in batches of KB: inverse y-FFT,
+// c2r along x, then per cell in kernel order  u = g/N; U_SUM += u; H += bell(u)*h
+// (mt_fft_step lenia.cpp:496-538).  Finally B = clamp(A + H*dt) (lenia.cpp:542-592).
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+struct InvCfg {
+    static constexpr int PLANE = H * (W / 2 + 1);  // float2 per spectrum plane
+    static constexpr int KB0 = (40 * 1024) / (PLANE * 8);
+    static constexpr int KB = KB0 < 1 ? 1 : (KB0 > LHD_MAX_K ? LHD_MAX_K : KB0);
+    static constexpr int THREADS = 256;
+};
+
+__device__ __forceinline__ float soft_growth(float g, float h, float dt) {
+    // soft clip lenia.cpp:549-571 with fastexp reintegration.h:16-37
+    const float a0 = 1.0f, a1 = 1.0f, a2 = 1.0f / 2, a3 = 1.0f / 2 / 3, a4 = 1.0f / 2 / 3 / 4;
+    const float a5 = 1.0f / 2 / 3 / 4 / 5, a6 = 1.0f / 2 / 3 / 4 / 5 / 6, a7 = 1.0f / 2 / 3 / 4 / 5 / 6 / 7;
+    float v = g + h * dt;
+    v = -4.0f * (v - 0.5f);
+    float r = a7 * v + a6;
+    r = r * v + a5; r = r * v + a4; r = r * v + a3; r = r * v + a2; r = r * v + a1; r = r * v + a0;
+    r = 1.0f + r;
+    r = 1.0f / r;
+    return clamp_ref(r, 0.0f, 1.0f);
+}
+
+template <int W, int H, int D>
+__global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    using Cfg = InvCfg<W, H, D>;
+    constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
+    __shared__ float2 sg[KB * Cfg::PLANE];
+    __shared__ float sh[LHD_MAX_C * PL];
+    __shared__ float sus[LHD_MAX_C * PL];
+    __shared__ float slut[KB * 256];
+    const int wi = blockIdx.x / D, z = blockIdx.x % D;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int tid = threadIdx.x, C = w.C, K = w.K;
+    const bool lut = (w.quirks & LHQ_GROWTH_LUT) != 0;
+    const float *A = A_cur(a, w);
+    for (int i = tid; i < C * PL; i += T) { sh[i] = 0.0f; sus[i] = 0.0f; }
+    const float2 *G = reinterpret_cast<const float2 *>(a.pool + w.Gsp);
+    float *sgf = reinterpret_cast<float *>(sg);
+    for (int k0 = 0; k0 < K; k0 += KB) {
+        const int nk = K - k0 < KB ? K - k0 : KB;
+        __syncthreads();
+        for (int i = tid; i < nk * Cfg::PLANE; i += T) {
+            const int kk = i / Cfg::PLANE, j = i % Cfg::PLANE;
+            sg[i] = G[(long long)(k0 + kk) * S::Nh + (long long)z * Cfg::PLANE + j];
+        }
+        if (lut)
+            for (int i = tid; i < nk * 256; i += T) slut[i] = a.pool[w.LUT + (long long)k0 * 256 + i];
+        __syncthreads();
+        for (int t = tid; t < nk * Wh; t += T) {
+            const int kk = t / Wh, kx = t % Wh;
+            float2 line[H];
+            float2 *p = sg + kk * Cfg::PLANE + kx;
+#pragma unroll
+            for (int y = 0; y < H; y++) line[y] = p[y * Wh];
+            fft_line<H, true>(line, a.twH);
+#pragma unroll
+            for (int y = 0; y < H; y++) p[y * Wh] = line[y];
+        }
+        __syncthreads();
+        for (int t = tid; t < nk * (H / 2); t += T) {
+            const int kk = t / (H / 2), y = 2 * (t % (H / 2));
+            const float2 *ra = sg + kk * Cfg::PLANE + y * Wh, *rb = ra + Wh;
+            float2 line[W];
+            line[0] = make_float2(ra[0].x, rb[0].x);
+            line[W / 2] = make_float2(ra[W / 2].x, rb[W / 2].x);
+#pragma unroll
+            for (int k = 1; k < W / 2; k++) {
+                const float2 av = ra[k], bv = rb[k];
+                line[k] = make_float2(av.x - bv.y, av.y + bv.x);
+                line[W - k] = make_float2(av.x + bv.y, bv.x - av.y);
+            }
+            fft_line<W, true>(line, a.twW);
+            float *u0 = sgf + 2 * (kk * Cfg::PLANE + y * Wh), *u1 = u0 + 2 * Wh;
+#pragma unroll
+            for (int x = 0; x < W; x++) { u0[x] = line[x].x; u1[x] = line[x].y; }
+        }
+        __syncthreads();
+        for (int i = tid; i < PL; i += T) {
+            const int y = i / W, x = i % W;
+            for (int kk = 0; kk < nk; kk++) {
+                const DevKernel &kp = w.kern[k0 + kk];
+                const int c = kp.c1;
+                const float u = sgf[2 * (kk * Cfg::PLANE + y * Wh) + x] / (float)S::N;
+                sus[c * PL + i] += u;
+                float g;
+                if (lut) {
+                    int iu = (int)(u * 255.0f);
+                    iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
+                    g = slut[kk * 256 + iu];
+                } else {
+                    const float q = (u - kp.m) / kp.s;
+                    g = expf(-(q * q) / 2.0f);
+                }
+                if ((w.asym_mask >> c) & 1) sh[c * PL + i] += (g - A[(long long)c * S::N + (long long)z * PL + i]) * kp.h;
+                else sh[c * PL + i] += g * kp.h;
+            }
+        }
+    }
+    __syncthreads();
+    float *Hg = a.pool + w.Hg, *B = a.pool + w.Bk, *US = a.pool + w.USUM;
+    for (int i = tid; i < C * PL; i += T) {
+        const int c = i / PL, j = i % PL;
+        const long long g = (long long)c * S::N + (long long)z * PL + j;
+        const float h = sh[i];
+        Hg[g] = h;
+        if (w.keep_usum) US[g] = sus[i];
+        if (w.growth_type == 1) {
+            B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(A[g], h, w.dt) : clamp_ref(A[g] + h * w.dt, 0.0f, 1.0f);
+        }
+    }
+}
+
+// inv_blur: inverse y-FFT + c2r of the C+1 blurred planes; U_S = g/N, SUM_S = g/N
+// (h_sum_blur reintegration.cpp:156-177).
+template <int W, int H, int D>
+__global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    using Cfg = InvCfg<W, H, D>;
+    constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
+    __shared__ float2 sg[KB * Cfg::PLANE];
+    const int wi = blockIdx.x / D, z = blockIdx.x % D;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int tid = threadIdx.x, NF = w.C + 1;
+    const float2 *G = reinterpret_cast<const float2 *>(a.pool + w.Gsp);
+    float *sgf = reinterpret_cast<float *>(sg);
+    for (int f0 = 0; f0 < NF; f0 += KB) {
+        const int nk = NF - f0 < KB ? NF - f0 : KB;
+        __syncthreads();
+        for (int i = tid; i < nk * Cfg::PLANE; i += T) {
+            const int kk = i / Cfg::PLANE, j = i % Cfg::PLANE;
+            sg[i] = G[(long long)(f0 + kk) * S::Nh + (long long)z * Cfg::PLANE + j];
+        }
+        __syncthreads();
+        for (int t = tid; t < nk * Wh; t += T) {
+            const int kk = t / Wh, kx = t % Wh;
+            float2 line[H];
+            float2 *p = sg + kk * Cfg::PLANE + kx;
+#pragma unroll
+            for (int y = 0; y < H; y++) line[y] = p[y * Wh];
+            fft_line<H, true>(line, a.twH);
+#pragma unroll
+            for (int y = 0; y < H; y++) p[y * Wh] = line[y];
+        }
+        __syncthreads();
+        for (int t = tid; t < nk * (H / 2); t += T) {
+            const int kk = t / (H / 2), y = 2 * (t % (H / 2));
+            const float2 *ra = sg + kk * Cfg::PLANE + y * Wh, *rb = ra + Wh;
+            float2 line[W];
+            line[0] = make_float2(ra[0].x, rb[0].x);
+            line[W / 2] = make_float2(ra[W / 2].x, rb[W / 2].x);
+#pragma unroll
+            for (int k = 1; k < W / 2; k++) {
+                const float2 av = ra[k], bv = rb[k];
+                line[k] = make_float2(av.x - bv.y, av.y + bv.x);
+                line[W - k] = make_float2(av.x + bv.y, bv.x - av.y);
+            }
+            fft_line<W, true>(line, a.twW);
+            float *u0 = sgf + 2 * (kk * Cfg::PLANE + y * Wh), *u1 = u0 + 2 * Wh;
+#pragma unroll
+            for (int x = 0; x < W; x++) { u0[x] = line[x].x; u1[x] = line[x].y; }
+        }
+        __syncthreads();
+        for (int i = tid; i < nk * PL; i += T) {
+            const int kk = i / PL, j = i % PL, y = j / W, x = j % W;
+            const int f = f0 + kk;
+            const float v = sgf[2 * (kk * Cfg::PLANE + y * Wh) + x] / (float)S::N;
+            float *dst = f < w.C ? a.pool + w.US + (long long)f * S::N : a.pool + w.SUMS;
+            dst[(long long)z * PL + j] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// local: per cell.  Gradients of S and H_c (Sobel with the reference's axis order, or
+// central differences), alpha (with the PREALPHA index quirk), F, mu, V.
+// ------------------------------------------------------------------------------------
+struct FieldRef {
+    const float *p;   // interior values [N]
+    long long cut;    // padded flat index bound: interior cells at or above read 0
+};
+
+template <int W, int H, int D>
+__device__ __forceinline__ float field_at(const FieldRef &f, int x, int y, int z, int kind, int faces) {
+    using S = Shape<W, H, D>;
+    if (!halo_src<W, H, D>(x, y, z, kind, faces)) return 0.0f;
+    if (S::pidx(x, y, z) >= f.cut) return 0.0f;
+    return f.p[x + W * (y + H * z)];
+}
+
+template <int W, int H, int D, bool PERM>
+__device__ __forceinline__ void sobel3(const FieldRef &f, int x, int y, int z, int kind, int faces, float (&g)[3]) {
+    constexpr int SW[3] = {1, 2, 1};
+    constexpr int SD[3] = {-1, 0, 1};
+    g[0] = 0.0f; g[1] = 0.0f; g[2] = 0.0f;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int gx = SD[dx + 1] * SW[dy + 1] * SW[dz + 1];
+                const int gy = SW[dx + 1] * SD[dy + 1] * SW[dz + 1];
+                const int gz = SW[dx + 1] * SW[dy + 1] * SD[dz + 1];
+                const int c0 = PERM ? gz : gx, c1 = PERM ? gx : gy, c2 = PERM ? gy : gz;
+                if (c0 == 0 && c1 == 0 && c2 == 0) continue;
+                const float v = field_at<W, H, D>(f, x + dx, y + dy, z + dz, kind, faces);
+                if (c0 != 0) g[0] = g[0] + v * (float)c0;
+                if (c1 != 0) g[1] = g[1] + v * (float)c1;
+                if (c2 != 0) g[2] = g[2] + v * (float)c2;
+            }
+}
+
+template <int W, int H, int D>
+__device__ __forceinline__ void cdiff3(const FieldRef &f, int x, int y, int z, int kind, int faces, float (&g)[3]) {
+    g[0] = (field_at<W, H, D>(f, x + 1, y, z, kind, faces) - field_at<W, H, D>(f, x - 1, y, z, kind, faces)) * 0.5f;
+    g[1] = (field_at<W, H, D>(f, x, y + 1, z, kind, faces) - field_at<W, H, D>(f, x, y - 1, z, kind, faces)) * 0.5f;
+    g[2] = (field_at<W, H, D>(f, x, y, z + 1, kind, faces) - field_at<W, H, D>(f, x, y, z - 1, kind, faces)) * 0.5f;
+}
+
+template <int W, int H, int D>
+__device__ __forceinline__ void gradient(const DevWorld &w, const FieldRef &f, int x, int y, int z, float (&g)[3]) {
+    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+    if (w.gradient == 1) {
+        if (w.quirks & LHQ_SOBEL_PERMUTE) sobel3<W, H, D, true>(f, x, y, z, w.sobel_border, faces, g);
+        else sobel3<W, H, D, false>(f, x, y, z, w.sobel_border, faces, g);
+    } else {
+        cdiff3<W, H, D>(f, x, y, z, w.sobel_border, faces, g);
+    }
+}
+
+template <int W, int H, int D>
+__global__ __launch_bounds__(256) void k_local(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    constexpr int BPW = (S::N + 255) / 256;
+    const int wi = blockIdx.x / BPW;
+    const int i = (blockIdx.x % BPW) * 256 + threadIdx.x;
+    if (i >= S::N) return;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int x = i % W, y = (i / W) % H, z = i / (W * H);
+    const int C = w.C;
+    const float *A = A_cur(a, w);
+    const long long BIG = 0x7FFFFFFFFFFFFFFFLL;
+    // gradient sources: Sobel reads SUM and H; central differences read SUM_S and U_S
+    FieldRef fs;
+    const bool cd_copy = w.gradient == 0 && !w.blur;
+    if (w.gradient == 1 || !w.blur) fs = FieldRef{a.pool + w.S, BIG};
+    else fs = FieldRef{a.pool + w.SUMS, BIG};
+    float na[3];
+    gradient<W, H, D>(w, fs, x, y, z, na);
+    const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
+    const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+    const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
+    const bool wall = w.border == 1;
+    const bool qa = (w.quirks & LHQ_PREALPHA_INDEX) != 0;
+    for (int c = 0; c < C; c++) {
+        FieldRef fh;
+        if (w.gradient == 1) fh = FieldRef{a.pool + w.Hg + (long long)c * S::N, BIG};
+        else if (w.blur) fh = FieldRef{a.pool + w.US + (long long)c * S::N, BIG};
+        else {
+            // U_S <- H by two overlapping copies covering the first 5P/2 padded floats
+            const long long cut = (w.quirks & LHQ_CD_COPY) ? (5 * S::P / 2) - (long long)c * S::P : BIG;
+            fh = FieldRef{a.pool + w.Hg + (long long)c * S::N, cut};
+        }
+        (void)cd_copy;
+        float nu[3];
+        gradient<W, H, D>(w, fh, x, y, z, nu);
+        // alpha (lenia.cpp:647-672, read lenia.cpp:708)
+        float al;
+        {
+            long long r = (long long)c * S::N + i;
+            int cs = c;
+            long long is = i;
+            bool ok = true;
+            if (qa) { cs = (int)(r / S::P); is = r - (long long)cs * S::P; ok = cs < C && is < S::N; }
+            if (ok) {
+                float pg = A[(long long)cs * S::N + is] / w.theta_a;
+                pg = pg * pg;
+                al = clamp_ref(pg, 0.0f, 1.0f);
+            } else {
+                al = 0.0f;
+            }
+        }
+        const float al1 = 1.0f - al;
+        float F[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
+        // mu and velocity (reintegration.cpp:321-602)
+        float *V = a.pool + w.V + (long long)c * 3 * S::N;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (w.v_live) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
+        }
+        if (w.mu_vel) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
+        }
+        float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
+        if (w.mode == 2) {
+            // advection displacement clamp(V*dt or F*dt, -ma, ma) (reintegration.cpp:1142-1166)
+            float vn[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float src = w.adv_vel ? vn[d] : F[d];
+                MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                float m = (w.mu_vel ? v[d] : F[d]) * dt;
+                m = fmax_ref(m, nma);
+                m = fmin_ref(m, ma);
+                m = pos[d] + m;
+                if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
+                MU[(long long)d * S::N + i] = m;
+            }
+        }
+        if (!w.mu_vel) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
+        }
+        if (w.v_live) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
+        }
+        if (w.need_f) {
+            float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
+#pragma unroll
+            for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// transport: per destination cell.  SRT / RT gather (27 sources for dd=1) or trilinear
+// advection into TT, then the end-of-step TRM: clamp, A <- TT, GA, SUM.
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+__global__ __launch_bounds__(256) void k_transport(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    constexpr int BPW = (S::N + 255) / 256;
+    const int wi = blockIdx.x / BPW;
+    const int i = (blockIdx.x % BPW) * 256 + threadIdx.x;
+    if (i >= S::N) return;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int x = i % W, y = (i / W) % H, z = i / (W * H);
+    const int C = w.C;
+    const float *Acur = A_cur(a, w);
+    const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? Acur : a.pool + w.Bk;
+    const float *MU = a.pool + w.MU;
+    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+    const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
+    const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
+    const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+    float tt[LHD_MAX_C] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (w.mode == 0) {
+#pragma unroll
+        for (int c = 0; c < LHD_MAX_C; c++) {
+            if (c >= nrt) break;
+            const float *mu = MU + (long long)c * 3 * S::N;
+            float m[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                float dv = pos[d] - mu[(long long)d * S::N + i];
+                dv = dv < 0.0f ? -dv : dv;
+                dv = w.si - dv;
+                m[d] = fmin_ref(fmax_ref(dv, 0.0f), w.clip);
+            }
+            float wt = ARR[(long long)c * S::N + i] * m[0];
+            wt = wt * m[1];
+            wt = wt * m[2];
+            wt = wt * w.sn;
+            tt[c] = tt[c] + wt;
+        }
+    } else if (w.mode == 1) {
+        const bool torus = w.border == 0;
+        const float ext[3] = {(float)W, (float)H, (float)D};
+        const float half[3] = {(float)(W / 2), (float)(H / 2), (float)(D / 2)};
+        const int dd = w.dd;
+        for (int dz = -dd; dz <= dd; dz++)
+            for (int dy = -dd; dy <= dd; dy++)
+                for (int dx = -dd; dx <= dd; dx++) {
+                    int sx = x + dx, sy = y + dy, sz = z + dz;
+                    if (!halo_src<W, H, D>(sx, sy, sz, tkind, faces)) continue;
+                    const long long s = sx + W * (sy + H * sz);
+#pragma unroll
+                    for (int c = 0; c < LHD_MAX_C; c++) {
+                        if (c >= nrt) break;
+                        const float *mu = MU + (long long)c * 3 * S::N;
+                        float m[3];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            float dv = pos[d] - mu[(long long)d * S::N + s];
+                            if (torus) {
+                                if (dv > half[d]) dv = dv - ext[d];
+                                if (dv < -half[d]) dv = dv + ext[d];
+                            }
+                            dv = dv < 0.0f ? -dv : dv;
+                            dv = w.si - dv;
+                            dv = fmax_ref(dv, 0.0f);
+                            m[d] = fmin_ref(dv, w.clip);
+                        }
+                        float wt = ARR[(long long)c * S::N + s] * m[0];
+                        wt = wt * m[1];
+                        wt = wt * m[2];
+                        wt = wt * w.sn;
+                        tt[c] = tt[c] + wt;
+                    }
+                }
+    } else {
+        const float ext[3] = {(float)W, (float)H, (float)D};
+        const bool wrap = (w.quirks & LHQ_TRANSPORT_TORUS) || w.border == 0;
+#pragma unroll
+        for (int c = 0; c < LHD_MAX_C; c++) {
+            if (c >= C) break;
+            const float *dl = MU + (long long)c * 3 * S::N;
+            int I0[3];
+            float F0[3], F1[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                float p = (pos[d] - 0.5f) - dl[(long long)d * S::N + i];
+                if (wrap) {
+                    p = p + ext[d];
+                    p = p - floorf(p / ext[d]) * ext[d];
+                }
+                const float fl = floorf(p);
+                I0[d] = (int)fl;
+                F1[d] = p - fl;
+                F0[d] = 1.0f - F1[d];
+            }
+            const float *Ac = ARR + (long long)c * S::N;
+            float acc = 0.0f;
+#pragma unroll
+            for (int corner = 0; corner < 8; corner++) {
+                const int bx = corner & 1, by = (corner >> 1) & 1, bz = (corner >> 2) & 1;
+                int cx = I0[0] + bx, cy = I0[1] + by, cz = I0[2] + bz;
+                float v = 0.0f;
+                if (halo_src<W, H, D>(cx, cy, cz, tkind, faces)) v = Ac[cx + W * (cy + H * cz)];
+                float t = v * (bx ? F1[0] : F0[0]);
+                t = t * (by ? F1[1] : F0[1]);
+                t = t * (bz ? F1[2] : F0[2]);
+                acc = corner == 0 ? t : acc + t;
+            }
+            tt[c] = acc;
+        }
+    }
+    // TRM (lenia.cpp:930-1035)
+    const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
+    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
+    const float *Hg = a.pool + w.Hg;
+    float ttsum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < LHD_MAX_C; c++) {
+        if (c >= C) break;
+        const long long g = (long long)c * S::N + i;
+        float t = tt[c];
+        if (w.growth_type == 2) {
+            const bool soft = (w.soft_mask >> c) & 1;
+            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) a.pool[w.Bk + g] = soft_growth(Acur[g], Hg[g], w.dt);
+            else if (soft) t = soft_growth(t, Hg[g], w.dt);
+            else t = clamp01 ? clamp_ref(t + Hg[g] * w.dt, 0.0f, 1.0f) : t + Hg[g] * w.dt;
+        } else if (clamp01) {
+            t = clamp_ref(t, 0.0f, 1.0f);
+        }
+        ttsum = ttsum + t;
+        GA[g] = w.gtam * GA[g] + t * w.gtas;
+        An[g] = t;
+    }
+    const float sold = (w.quirks & LHQ_SUM_RESET) ? 0.0f : Sg[i];
+    Sg[i] = w.stam * sold + ttsum * w.stas;
+}
+
+// ------------------------------------------------------------------------------------
+// Shape table.
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+struct Launch {
+    // arg = source | (max fields per world) << 8
+    static void fwd_xy(const PhaseArgs &a, int arg, void *st) {
+        const int grid = a.nworlds * (arg >> 8) * (D / FwdCfg<W, H, D>::NP);
+        hipLaunchKernelGGL((k_fwd_xy<W, H, D>), dim3(grid), dim3(128), 0, (hipStream_t)st, a, arg);
+    }
+    static void spec_z(const PhaseArgs &a, int mode, void *st) {
+        constexpr int BPW = ((W / 2 + 1) * H + 127) / 128;
+        hipLaunchKernelGGL((k_spec_z<W, H, D>), dim3(a.nworlds * BPW), dim3(128), 0, (hipStream_t)st, a, mode);
+    }
+    static void kspec_z(const PhaseArgs &a, int, void *st) {
+        constexpr int BPW = ((W / 2 + 1) * H + 127) / 128;
+        hipLaunchKernelGGL((k_kspec_z<W, H, D>), dim3(a.nworlds * BPW), dim3(128), 0, (hipStream_t)st, a, 0);
+    }
+    static void inv_growth(const PhaseArgs &a, int, void *st) {
+        hipLaunchKernelGGL((k_inv_growth<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
+    }
+    static void inv_blur(const PhaseArgs &a, int, void *st) {
+        hipLaunchKernelGGL((k_inv_blur<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
+    }
+    static void local(const PhaseArgs &a, int, void *st) {
+        constexpr int BPW = (W * H * D + 255) / 256;
+        hipLaunchKernelGGL((k_local<W, H, D>), dim3(a.nworlds * BPW), dim3(256), 0, (hipStream_t)st, a, 0);
+    }
+    static void transport(const PhaseArgs &a, int, void *st) {
+        constexpr int BPW = (W * H * D + 255) / 256;
+        hipLaunchKernelGGL((k_transport<W, H, D>), dim3(a.nworlds * BPW), dim3(256), 0, (hipStream_t)st, a, 0);
+    }
+    static constexpr ShapeOps ops() {
+        return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z};
+    }
+};
+
+#define LH_SHAPES(X) \
+    X(8, 8, 8)       \
+    X(8, 4, 8)       \
+    X(12, 12, 12)    \
+    X(16, 16, 16)    \
+    X(16, 8, 16)     \
+    X(24, 24, 24)    \
+    X(32, 32, 32)    \
+    X(36, 36, 36)    \
+    X(40, 40, 40)    \
+    X(44, 24, 44)    \
+    X(48, 48, 48)
+
+#define LH_OPS_ENTRY(w, h, d) Launch<w, h, d>::ops(),
+static const ShapeOps g_ops[] = {LH_SHAPES(LH_OPS_ENTRY)};
+
+const ShapeOps *lh_shape_ops(int W, int H, int D) {
+    for (const ShapeOps &o : g_ops)
+        if (o.W == W && o.H == H && o.D == D) return &o;
+    return nullptr;
+}
+
+// ------------------------------------------------------------------------------------
+// Shape-independent kernels: FP64 mass per world, marching-cubes case index.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mass(PhaseArgs a, double *out) {
+    const DevWorld &w = a.worlds[blockIdx.x];
+    const long long n = (long long)w.C * w.W * w.H * w.D;
+    const float *A = a.pool + (a.parity ? w.A2 : w.A);
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 256) s += (double)A[i];
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Case index of cell (x,y,z): bit i set iff the value at corner i (mesh.cpp:19-32)
+// is > iso (mesh.cpp:173-180); corners past the far faces read the torus face pad,
+// edge/corner pads read 0 when face_only (transposition.cpp:881-995).
+__global__ __launch_bounds__(256) void k_mc_case(const float *f, int W, int H, int D, float iso,
+                                                 int face_only, unsigned char *out) {
+    const long long n = (long long)W * H * D;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / ((long long)W * H));
+    const int DX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    const int DY[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    const int DZ[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int cx = x + DX[k], cy = y + DY[k], cz = z + DZ[k];
+        const int m = (cx >= W) + (cy >= H) + (cz >= D);
+        float val = 0.0f;
+        if (m <= 1 || !face_only) {
+            cx = cx >= W ? cx - W : cx;
+            cy = cy >= H ? cy - H : cy;
+            cz = cz >= D ? cz - D : cz;
+            val = f[cx + (long long)W * (cy + (long long)H * cz)];
+        }
+        if (val > iso) v |= 1 << k;
+    }
+    out[i] = (unsigned char)v;
+}
+
+void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st) {
+    hipLaunchKernelGGL(k_mass, dim3(nworlds), dim3(256), 0, (hipStream_t)st, a, out);
+}
+
+void lh_launch_mc(const float *f, int W, int H, int D, float iso, int face_only, unsigned char *out, void *st) {
+    const long long n = (long long)W * H * D;
+    hipLaunchKernelGGL(k_mc_case, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)st, f, W, H, D,
+                       iso, face_only, out);
+}

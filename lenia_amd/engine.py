@@ -1,0 +1,122 @@
+"""Python front-end of the MI355X Flow Lenia engine (over the C-ABI in lenia_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Engine:
+    """A batch of worlds resident in HBM, advanced together on one HIP stream.
+
+    >>> eng = Engine(); wid = eng.add_world(shape, C, kernels, cfg); eng.commit()
+    >>> eng.set(wid, "A", A0); eng.step(100); A = eng.get(wid, "A")
+    """
+
+    FIELDS = {"A": L.F_A, "GA": L.F_GA, "S": L.F_S, "V": L.F_V, "H": L.F_H, "B": L.F_B,
+              "MU": L.F_MU, "USUM": L.F_USUM}
+
+    def __init__(self, device: int = 0, keep_velocity=False, keep_usum=False, graph=True,
+                 profile=False):
+        self._L = L.load()
+        h = C.c_void_p()
+        L.check(self._L.lh_create(device, C.byref(h)), "lh_create")
+        self._h = h
+        self.worlds = []
+        L.check(self._L.lh_set_option(self._h, L.OPT_KEEP_VELOCITY, int(keep_velocity)), "keep_velocity")
+        L.check(self._L.lh_set_option(self._h, L.OPT_KEEP_USUM, int(keep_usum)), "keep_usum")
+        self.set_option(L.OPT_USE_GRAPH, graph)
+        self.set_option(L.OPT_PROFILE, profile)
+
+    def set_option(self, opt, value):
+        L.check(self._L.lh_set_option(self._h, opt, int(value)), "lh_set_option")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_world(self, shape, C_, kernels, cfg) -> int:
+        W, H, D = shape
+        s = L.Shape(W, H, D, C_, len(kernels))
+        p = L.make_params(cfg)
+        k = L.make_kernels(kernels)
+        wid = C.c_int()
+        L.check(self._L.lh_add_world(self._h, C.byref(s), C.byref(p), k, C.byref(wid)), "lh_add_world")
+        self.worlds.append((tuple(shape), C_, len(kernels)))
+        return wid.value
+
+    def commit(self):
+        L.check(self._L.lh_commit(self._h), "lh_commit")
+
+    def _count(self, wid, field):
+        (W, H, D), C_, _ = self.worlds[wid]
+        N = W * H * D
+        return {"A": C_ * N, "GA": C_ * N, "S": N, "V": 3 * C_ * N, "H": C_ * N, "B": C_ * N,
+                "MU": 3 * C_ * N, "USUM": C_ * N}[field]
+
+    def set(self, wid, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(-1)
+        if a.size != self._count(wid, field):
+            raise ValueError(f"{field}: expected {self._count(wid, field)} floats, got {a.size}")
+        L.check(self._L.lh_set_field(self._h, wid, self.FIELDS[field],
+                                     a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "lh_set_field")
+
+    def get(self, wid, field) -> np.ndarray:
+        a = np.empty(self._count(wid, field), dtype=np.float32)
+        L.check(self._L.lh_get_field(self._h, wid, self.FIELDS[field],
+                                     a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "lh_get_field")
+        return a
+
+    def device_ptr(self, wid, field) -> int:
+        p = C.POINTER(C.c_float)()
+        n = C.c_size_t()
+        L.check(self._L.lh_field_device_ptr(self._h, wid, self.FIELDS[field], C.byref(p), C.byref(n)),
+                "lh_field_device_ptr")
+        return C.cast(p, C.c_void_p).value
+
+    def state_updated(self):
+        L.check(self._L.lh_state_updated(self._h), "lh_state_updated")
+
+    def step(self, n=1):
+        L.check(self._L.lh_step(self._h, int(n)), "lh_step")
+
+    def synchronize(self):
+        L.check(self._L.lh_synchronize(self._h), "lh_synchronize")
+
+    def stream(self) -> int:
+        return self._L.lh_stream(self._h)
+
+    def mass(self) -> np.ndarray:
+        out = np.empty(len(self.worlds), dtype=np.float64)
+        L.check(self._L.lh_world_mass(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size),
+                "lh_world_mass")
+        return out
+
+    def mc_case(self, wid, field="A", channel=0, iso=0.5, face_only=True) -> np.ndarray:
+        (W, H, D), _, _ = self.worlds[wid]
+        out = np.empty(W * H * D, dtype=np.uint8)
+        L.check(self._L.lh_mc_case(self._h, wid, self.FIELDS[field], channel, float(iso), int(face_only),
+                                   out.ctypes.data_as(C.POINTER(C.c_uint8))), "lh_mc_case")
+        return out.reshape(D, H, W)
+
+    def profile(self) -> dict:
+        out = {}
+        for i in range(self._L.lh_profile_count(self._h)):
+            name, n, ms = C.c_char_p(), C.c_long(), C.c_double()
+            L.check(self._L.lh_profile_entry(self._h, i, C.byref(name), C.byref(n), C.byref(ms)),
+                    "lh_profile_entry")
+            out[name.value.decode()] = (n.value, ms.value)
+        return out
+
+    def profile_reset(self):
+        L.check(self._L.lh_profile_reset(self._h), "lh_profile_reset")
+
+
+def shape_supported(W, H, D) -> bool:
+    return bool(L.load().lh_shape_supported(W, H, D))

@@ -1,0 +1,5 @@
+/* Double-precision instantiation of the CPU oracle: the FP64 reference for error
+ * statements (test infrastructure only). */
+#define REAL double
+#define PFX(n) lo_f64_##n
+#include "lenia_oracle_impl.h"
