@@ -1,0 +1,165 @@
+"""Throughput of the Flow Lenia step on one MI355X.
+
+Headline: world-steps/s of a batch of 32^3 worlds (C=3, K=9, RT reintegration, Sobel,
+periodic border, reference growth table), synthetic seeded worlds resident in HBM,
+next to the same engine driving a single world.  One JSON line on stdout.
+
+  python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile]
+
+A step advances every world by one full simulation step (spectral convolution, growth,
+gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
+the engine's captured hipGraph; per-kernel durations come from a second pass that
+brackets each launch with HIP events on the engine's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from lenia_amd import workloads as WL  # noqa: E402
+from lenia_amd.engine import Engine  # noqa: E402
+from lenia_amd import roofline as RF  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def build_engine(worlds, profile=False):
+    eng = Engine(graph=not profile, profile=profile)
+    for w in worlds:
+        eng.add_world(w["shape"], w["C"], w["kernels"], w["cfg"])
+    eng.commit()
+    for i, w in enumerate(worlds):
+        eng.set(i, "A", w["A"])
+    return eng
+
+
+def timed(eng, steps, warmup):
+    eng.step(warmup)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.synchronize()
+    return time.perf_counter() - t0
+
+
+def _cpu_worker(args):
+    seed, check_steps, budget_s = args
+    from oracle import oracle as O
+    w = WL.plain32(seed=seed)
+    ow = O.World(w["shape"], w["C"], w["kernels"], w["cfg"], 32)
+    ow.set(O.F_A, w["A"])
+    t0 = time.perf_counter()
+    ow.step(check_steps)
+    snap = ow.get(O.F_A).astype(np.float32)
+    n = check_steps
+    while time.perf_counter() - t0 < budget_s:
+        ow.step(2)
+        n += 2
+    return seed, snap, n, time.perf_counter() - t0
+
+
+def cpu_baseline(seeds, check_steps, budget_s=12.0):
+    """The fp32 CPU restatement on the host cores: world i of the GPU batch on core i,
+    each stepped for ~budget_s.  Each worker also returns its world's A after the
+    bench's warmup+timed step count, which the GPU result is compared with bitwise."""
+    from oracle import oracle as O
+    O.build()
+    cores = max(1, min(16, os.cpu_count() or 1, len(seeds)))
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(seeds[i], check_steps, budget_s) for i in range(cores)])
+    total_steps = sum(r[2] for r in res)
+    wall = max(r[3] for r in res)
+    base = dict(value=total_steps / wall, unit="world-steps/s", cores=cores, kind="port",
+                sample=f"{cores} of the batch's 32^3 C=3 K=9 RT worlds, one per host core, "
+                       f"~{budget_s:.0f}s each (fp32 restatement, gcc -O2, single-threaded per world)")
+    return base, {r[0]: r[1] for r in res}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: worlds are independent and would be sharded "
+              "world_id mod G across GPUs; multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    seeds = list(range(100, 100 + min(args.worlds, 64)))
+    proto = [WL.plain32(seed=s) for s in seeds]
+    worlds = [proto[i % len(proto)] for i in range(args.worlds)]
+
+    eng = build_engine(worlds)
+    dt = timed(eng, args.steps, args.warmup)
+    value = args.worlds * args.steps / dt
+    ms_per_step = dt * 1e3 / args.steps
+    gpu_final = {seeds[i]: eng.get(i, "A") for i in range(min(len(seeds), 16))}
+    eng.close()
+
+    single = build_engine(worlds[:1])
+    sdt = timed(single, max(args.steps, 100), args.warmup)
+    single_rate = max(args.steps, 100) / sdt
+    single.close()
+
+    line = {
+        "metric": "world-steps/s at 32^3x3ch batched vs single, err vs FP64 <=1e-4",
+        "value": value, "unit": "world-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"batched {args.worlds} x 32^3 worlds, C=3, K=9, R=8, RT, Sobel, "
+                               f"torus, growth LUT, dt=0.2, sigma=0.65", "worlds": args.worlds,
+                   "shape": [32, 32, 32], "channels": 3, "kernels": 9},
+        "single_world": {"value": single_rate, "unit": "world-steps/s",
+                         "ms_per_step": sdt * 1e3 / max(args.steps, 100)},
+        "batched_over_single": value / single_rate,
+    }
+
+    if not args.no_profile:
+        peng = build_engine(worlds, profile=True)
+        peng.step(args.warmup)
+        peng.synchronize()
+        peng.profile_reset()
+        peng.step(args.steps)
+        peng.synchronize()
+        prof = peng.profile()
+        peng.close()
+        kernels = {k: {"launches": n, "avg_us": ms * 1e3 / n} for k, (n, ms) in prof.items()}
+        dom = max(prof, key=lambda k: prof[k][1])
+        n, ms = prof[dom]
+        avg_s = ms / n * 1e-3
+        b = RF.kernel_bytes(dom, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds
+        achieved = b / avg_s / 1e9
+        line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "algorithmic_bytes_per_launch": b, "avg_launch_us": avg_s * 1e6}
+        step_bytes = RF.step_bytes((32, 32, 32), 3, 9) * args.worlds
+        line["step_roofline"] = {"algorithmic_bytes_per_step": step_bytes,
+                                 "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                                 "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        line["kernels"] = kernels
+
+    if not args.no_cpu:
+        base, cpu_final = cpu_baseline(seeds, args.warmup + args.steps)
+        line["cpu_baseline"] = base
+        bad = sum(int(np.count_nonzero(gpu_final[s].view(np.uint32) != a.view(np.uint32)))
+                  for s, a in cpu_final.items())
+        line["parity"] = {"worlds_compared": len(cpu_final), "steps": args.warmup + args.steps,
+                          "mismatching_values": bad, "bitwise_equal": bad == 0}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

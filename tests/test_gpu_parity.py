@@ -1,0 +1,220 @@
+"""GPU parity: the HIP step engine (through the C-ABI) against the CPU checker.
+
+Bar: bit-identical fp32 results for the reference-compatible step (growth table,
+reference halos and index quirks); 1e-4 max-abs per channel against the fp64 checker
+where growth is analytic; FP64 per-world mass conserved to 1e-5 relative in physics mode.
+"""
+import numpy as np
+import pytest
+
+from lenia_amd import workloads as WL
+from tests.pairs import mismatch, run_pair
+
+pytestmark = pytest.mark.gpu
+
+ASYM = [1, 1, 1, 1]
+
+CASES = {
+    "rt_sobel_torus": dict(seed=1),
+    "rt_asymptotic": dict(seed=2, asymptotic=ASYM),
+    "srt": dict(seed=3, mode="srt", asymptotic=ASYM),
+    "adv_vel": dict(seed=4, mode="adv", asymptotic=ASYM),
+    "adv_force": dict(seed=14, mode="adv", adv_vel=False, asymptotic=ASYM),
+    "cd_blur_zero": dict(seed=5, gradient="central", blur=True, sobel_border="none", border="none",
+                         asymptotic=ASYM),
+    "cd_copy_torus": dict(seed=6, gradient="central", blur=False, asymptotic=ASYM),
+    "cd_copy_wall": dict(seed=16, gradient="central", blur=False, sobel_border="wall", asymptotic=ASYM),
+    "wall_mu_vel": dict(seed=7, border="wall", sobel_border="wall", mu_vel=True, asymptotic=ASYM),
+    "sobel_none_rt_other": dict(seed=17, sobel_border="none", border="none", asymptotic=ASYM),
+    "growth_transposition": dict(seed=18, growth="transposition", asymptotic=ASYM),
+    "soft_clip": dict(seed=19, soft_clip=[1, 0, 1, 0], asymptotic=[0, 1, 1, 0]),
+    "soft_clip_transposition": dict(seed=20, growth="transposition", soft_clip=[1, 1, 0, 0]),
+    "growth_none_dd2": dict(seed=21, growth="none", dd=2.0, sigma=0.8, asymptotic=ASYM),
+    "s_norm_half": dict(seed=22, s_norm=0.5, sigma=0.55, asymptotic=ASYM),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_one_step_all_fields_bitwise(name):
+    w = WL.plain32(**CASES[name])
+    fields = ["A", "GA", "S", "V", "H"]
+    if w["cfg"]["growth"] == "rt":
+        fields.append("B")
+    if w["cfg"]["mode"] != "adv":
+        fields.append("MU")
+    r = run_pair(w, 1, fields=fields)
+    for f in fields:
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{name}: field {f} differs in {bad} values (max abs {err:.3e})"
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_ten_steps_bitwise(name):
+    w = WL.plain32(**CASES[name])
+    r = run_pair(w, 10, fields=["A", "S"])
+    for f in ("A", "S"):
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{name}: {f} after 10 steps differs in {bad} values (max abs {err:.3e})"
+
+
+@pytest.mark.parametrize("shape,C,K,R", [((8, 8, 8), 3, 4, 2), ((8, 4, 8), 2, 3, 2), ((12, 12, 12), 2, 3, 3),
+                                         ((16, 16, 16), 4, 6, 4), ((16, 8, 16), 3, 5, 3),
+                                         ((24, 24, 24), 3, 9, 6), ((36, 36, 36), 3, 9, 8),
+                                         ((40, 40, 40), 3, 9, 8), ((44, 24, 44), 3, 9, 8),
+                                         ((48, 48, 48), 4, 16, 8)])
+def test_shapes_bitwise(shape, C, K, R):
+    w = WL.world(shape, C, K, R, seed=sum(shape), asymptotic=ASYM)
+    r = run_pair(w, 5, fields=["A", "H", "MU"])
+    for f in ("A", "H", "MU"):
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
+
+
+@pytest.mark.parametrize("over", [{}, {"asymptotic": ASYM}])
+def test_config0_100_steps(over):
+    """32^3, C=3, K=9, RT, Sobel, torus, growth table: 100 steps, bit-identical
+    (so the max-abs error per channel is 0 <= 1e-4)."""
+    w = WL.plain32(seed=1, **over)
+    r = run_pair(w, 100, fields=["A"], keep_velocity=False)
+    gpu, ref = r["A"]
+    bad, err = mismatch(gpu, ref)
+    per_channel = np.abs(gpu.reshape(3, -1).astype(np.float64) - ref.reshape(3, -1)).max(axis=1)
+    assert bad == 0 and (per_channel <= 1e-4).all(), (bad, per_channel)
+
+
+def test_physics_mode_vs_fp64_one_step():
+    w = WL.plain32(seed=9, quirks=0, growth="none")
+    r = run_pair(w, 1, fields=["A"], keep_velocity=False, prec=64)
+    gpu, ref = r["A"]
+    assert np.abs(gpu.astype(np.float64) - ref).reshape(3, -1).max(axis=1).max() <= 1e-4
+
+
+def test_physics_mode_mass_conservation():
+    from lenia_amd.engine import Engine
+    w = WL.plain32(seed=4, quirks=0, growth="none")
+    eng = Engine(keep_velocity=False)
+    wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    eng.set(wid, "A", w["A"])
+    m0 = eng.mass()[0]
+    assert np.isclose(m0, w["A"].astype(np.float64).sum(), rtol=1e-12)
+    eng.step(100)
+    m1 = eng.mass()[0]
+    eng.close()
+    assert abs(m1 - m0) / m0 <= 1e-5, (m0, m1)
+
+
+def test_graph_and_eager_identical():
+    from lenia_amd.engine import Engine
+    w = WL.plain32(seed=12, asymptotic=ASYM)
+    outs = []
+    for graph in (True, False):
+        eng = Engine(graph=graph)
+        wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        eng.set(wid, "A", w["A"])
+        for n in (3, 4, 1, 2):  # odd and even chunks, graph replays between eager steps
+            eng.step(n)
+            eng.synchronize()
+            outs.append(eng.get(wid, "A"))
+        eng.close()
+    half = len(outs) // 2
+    for a, b in zip(outs[:half], outs[half:]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_batched_mixed_shapes_match_single_worlds():
+    """A batch of mixed shapes and per-world SRT/RT equals each world run alone, and
+    spot-checked worlds equal the CPU checker."""
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    ws = WL.mixed(n_worlds=16, base_seed=300, asymptotic=ASYM)
+    eng = Engine()
+    for w in ws:
+        eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    for i, w in enumerate(ws):
+        eng.set(i, "A", w["A"])
+    eng.step(6)
+    eng.synchronize()
+    batch = [eng.get(i, "A") for i in range(len(ws))]
+    eng.close()
+    for i in (0, 1, 2, 3, 5, 10):
+        w = ws[i]
+        ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 32)
+        ow.set(O.F_A, w["A"])
+        ow.step(6)
+        bad, err = mismatch(batch[i], ow.get(O.F_A))
+        assert bad == 0, (i, w["shape"], w["cfg"]["mode"], bad, err)
+
+
+def test_set_field_midrun_and_state_roundtrip():
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    w = WL.plain32(seed=30, asymptotic=ASYM, mu_vel=True)
+    eng = Engine(keep_velocity=True)
+    wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    eng.set(wid, "A", w["A"])
+    eng.step(3)
+    A3, V3, GA3, S3 = (eng.get(wid, f) for f in ("A", "V", "GA", "S"))
+    ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 32)
+    for f, a in ((O.F_A, A3), (O.F_V, V3), (O.F_GA, GA3), (O.F_S, S3)):
+        ow.set(f, a)
+    eng.set(wid, "A", A3)  # re-set mid-run: spectra are rebuilt from the new state
+    eng.step(4)
+    ow.step(4)
+    for f, of in (("A", O.F_A), ("V", O.F_V), ("GA", O.F_GA), ("S", O.F_S)):
+        bad, err = mismatch(eng.get(wid, f), ow.get(of))
+        assert bad == 0, (f, bad, err)
+    eng.close()
+
+
+def test_mc_case_index_bitwise():
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    w = WL.plain32(seed=40, asymptotic=ASYM)
+    eng = Engine(keep_usum=True)
+    wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    eng.set(wid, "A", w["A"])
+    eng.step(5)
+    eng.synchronize()
+    for field in ("A", "H", "USUM"):
+        f = eng.get(wid, field).reshape(3, 32, 32, 32)
+        for c in range(3):
+            for face_only in (True, False):
+                gpu = eng.mc_case(wid, field, c, 0.5, face_only)
+                ref = O.mc_case(f[c], 0.5, face_only, 32)
+                assert np.array_equal(gpu, ref), (field, c, face_only)
+    eng.close()
+
+
+def test_usum_matches_checker():
+    w = WL.plain32(seed=41, asymptotic=ASYM)
+    r = run_pair(w, 2, fields=["USUM"], keep_usum=True)
+    assert mismatch(*r["USUM"])[0] == 0
+
+
+def test_rejects_bad_input():
+    from lenia_amd import _lib as L
+    from lenia_amd.engine import Engine
+    eng = Engine()
+    w = WL.plain32(seed=1)
+    with pytest.raises(L.LeniaHipError, match="not built"):
+        eng.add_world((30, 30, 30), 3, w["kernels"], w["cfg"])
+    bad = [dict(k) for k in w["kernels"]]
+    bad[0]["c0"] = 5
+    with pytest.raises(L.LeniaHipError, match="channel out of range"):
+        eng.add_world((32, 32, 32), 3, bad, w["cfg"])
+    with pytest.raises(L.LeniaHipError, match="sigma"):
+        eng.add_world((32, 32, 32), 3, w["kernels"], dict(w["cfg"], dd=0.5))
+    with pytest.raises(L.LeniaHipError):
+        eng.step(1)  # before commit
+    wid = eng.add_world((32, 32, 32), 3, w["kernels"], w["cfg"])
+    eng.commit()
+    with pytest.raises(ValueError):
+        eng.set(wid, "A", np.zeros(10, np.float32))
+    with pytest.raises(L.LeniaHipError, match="U_SUM"):
+        eng.get(wid, "USUM")
+    eng.close()
