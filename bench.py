@@ -4,7 +4,7 @@ Headline: world-steps/s of a batch of 32^3 worlds (C=3, K=9, RT reintegration, S
 periodic border, reference growth table), synthetic seeded worlds resident in HBM,
 next to the same engine driving a single world.  One JSON line on stdout.
 
-  python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile]
+  python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-world leg")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: worlds are independent and would be sharded "
@@ -109,10 +112,12 @@ def main():
     gpu_final = {seeds[i]: eng.get(i, "A") for i in range(min(len(seeds), 16))}
     eng.close()
 
-    single = build_engine(worlds[:1])
-    sdt = timed(single, max(args.steps, 100), args.warmup)
-    single_rate = max(args.steps, 100) / sdt
-    single.close()
+    single_rate, sdt = float("nan"), float("nan")
+    if not args.no_single:
+        single = build_engine(worlds[:1])
+        sdt = timed(single, max(args.steps, 100), args.warmup)
+        single_rate = max(args.steps, 100) / sdt
+        single.close()
 
     line = {
         "metric": "world-steps/s at 32^3x3ch batched vs single, err vs FP64 <=1e-4",
@@ -158,6 +163,11 @@ def main():
                   for s, a in cpu_final.items())
         line["parity"] = {"worlds_compared": len(cpu_final), "steps": args.warmup + args.steps,
                           "mismatching_values": bad, "bitwise_equal": bad == 0}
+    if args.copy_mib:
+        ceng = Engine()
+        ceng.calibration_copy(args.copy_mib << 20)
+        ceng.close()
+        line["copy_calibration_bytes"] = 2 * (args.copy_mib << 20)
     print(json.dumps(line), flush=True)
 
 

@@ -142,6 +142,10 @@ int lh_profile_count(void *engine);
 /* i-th kernel: name, launches and total milliseconds since the last reset. */
 lh_status lh_profile_entry(void *engine, int i, const char **name, long *launches, double *ms);
 lh_status lh_profile_reset(void *engine);
+/* Measurement aid, no counterpart in the reference: one plain device-to-device copy
+ * kernel moving `bytes` bytes (rounded down to 16) on the engine stream, synchronous;
+ * its FETCH_SIZE / WRITE_SIZE counters calibrate HBM-traffic readings. */
+lh_status lh_calibration_copy(void *engine, size_t bytes);
 /* Whether a shape is supported by this build. */
 int lh_shape_supported(int W, int H, int D);
 const char *lh_last_error(void);

@@ -27,7 +27,7 @@ Q_ALL = 0x7FF
 EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commit", "lh_set_field",
            "lh_get_field", "lh_field_device_ptr", "lh_state_updated", "lh_step", "lh_synchronize",
            "lh_stream", "lh_world_mass", "lh_mc_case", "lh_world_count", "lh_profile_count",
-           "lh_profile_entry", "lh_profile_reset", "lh_shape_supported", "lh_last_error", "lh_version")
+           "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version")
 
 
 class Params(C.Structure):
@@ -92,6 +92,7 @@ def load():
         "lh_profile_count": (i, [vp]),
         "lh_profile_entry": (i, [vp, i, C.POINTER(C.c_char_p), C.POINTER(C.c_long), C.POINTER(C.c_double)]),
         "lh_profile_reset": (i, [vp]),
+        "lh_calibration_copy": (i, [vp, st]),
         "lh_shape_supported": (i, [i, i, i]),
         "lh_last_error": (C.c_char_p, []),
         "lh_version": (C.c_char_p, []),

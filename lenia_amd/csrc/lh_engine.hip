@@ -17,6 +17,7 @@
 
 void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st);
 void lh_launch_mc(const float *f, int W, int H, int D, float iso, int face_only, unsigned char *out, void *st);
+void lh_launch_copy(const void *src, void *dst, size_t bytes, void *st);
 
 namespace {
 
@@ -189,8 +190,7 @@ void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg)
     hipEventElapsedTime(&ms, t0, t1);
     hipEventDestroy(t0);
     hipEventDestroy(t1);
-    std::string key = std::string(name) + "<" + std::to_string(a.nworlds) + "x" + std::to_string(e.classes.empty() ? 0 : 0) + ">";
-    key = name;
+    const std::string key = name;
     auto it = e.prof.find(key);
     if (it == e.prof.end()) {
         e.prof_names.push_back(key);
@@ -715,6 +715,27 @@ lh_status lh_profile_reset(void *p) {
     Engine *e = as_engine(p);
     if (!e) return fail(LH_EINVAL, "null engine");
     for (auto &kv : e->prof) kv.second = Prof{};
+    return LH_OK;
+}
+
+lh_status lh_calibration_copy(void *p, size_t bytes) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    bytes &= ~(size_t)15;
+    if (bytes == 0) return fail(LH_EINVAL, "bytes must be >= 16");
+    HIPCHK(hipSetDevice(e->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        hipFree(a);
+        return fail(LH_ENOMEM, "hipMalloc failed");
+    }
+    hipMemsetAsync(a, 0, bytes, e->stream);
+    lh_launch_copy(a, b, bytes, e->stream);
+    hipError_t err = hipStreamSynchronize(e->stream);
+    hipFree(a);
+    hipFree(b);
+    if (err != hipSuccess) return fail(LH_EHIP, hipGetErrorString(err));
     return LH_OK;
 }
 

@@ -250,16 +250,27 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using Cfg = InvCfg<W, H, D>;
     constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
+    constexpr int CPT = (PL + T - 1) / T;  // cells of the plane owned by each thread
     __shared__ float2 sg[KB * Cfg::PLANE];
-    __shared__ float sh[LHD_MAX_C * PL];
-    __shared__ float sus[LHD_MAX_C * PL];
     __shared__ float slut[KB * 256];
     const int wi = blockIdx.x / D, z = blockIdx.x % D;
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C, K = w.K;
     const bool lut = (w.quirks & LHQ_GROWTH_LUT) != 0;
-    const float *A = A_cur(a, w);
-    for (int i = tid; i < C * PL; i += T) { sh[i] = 0.0f; sus[i] = 0.0f; }
+    const float *A = A_cur(a, w) + (long long)z * PL;
+    // per-cell accumulators live in registers: h[j][c], U_SUM us[j][c], and A for the
+    // asymptotic form, for the CPT cells tid + j*T of this plane
+    float h[CPT][LHD_MAX_C], us[CPT][LHD_MAX_C], av[CPT][LHD_MAX_C];
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const int i = tid + j * T;
+#pragma unroll
+        for (int c = 0; c < LHD_MAX_C; c++) {
+            h[j][c] = 0.0f;
+            us[j][c] = 0.0f;
+            av[j][c] = (c < C && i < PL) ? A[(long long)c * S::N + i] : 0.0f;
+        }
+    }
     const float2 *G = reinterpret_cast<const float2 *>(a.pool + w.Gsp);
     float *sgf = reinterpret_cast<float *>(sg);
     for (int k0 = 0; k0 < K; k0 += KB) {
@@ -291,9 +302,9 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
             line[W / 2] = make_float2(ra[W / 2].x, rb[W / 2].x);
 #pragma unroll
             for (int k = 1; k < W / 2; k++) {
-                const float2 av = ra[k], bv = rb[k];
-                line[k] = make_float2(av.x - bv.y, av.y + bv.x);
-                line[W - k] = make_float2(av.x + bv.y, bv.x - av.y);
+                const float2 av2 = ra[k], bv = rb[k];
+                line[k] = make_float2(av2.x - bv.y, av2.y + bv.x);
+                line[W - k] = make_float2(av2.x + bv.y, bv.x - av2.y);
             }
             fft_line<W, true>(line, a.twW);
             float *u0 = sgf + 2 * (kk * Cfg::PLANE + y * Wh), *u1 = u0 + 2 * Wh;
@@ -301,13 +312,17 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
             for (int x = 0; x < W; x++) { u0[x] = line[x].x; u1[x] = line[x].y; }
         }
         __syncthreads();
-        for (int i = tid; i < PL; i += T) {
-            const int y = i / W, x = i % W;
-            for (int kk = 0; kk < nk; kk++) {
-                const DevKernel &kp = w.kern[k0 + kk];
-                const int c = kp.c1;
+        // growth, mt_fft_step lenia.cpp:501-538: per cell and output channel, kernels in order
+        for (int kk = 0; kk < nk; kk++) {
+            const DevKernel &kp = w.kern[k0 + kk];
+            const int c1 = kp.c1;
+            const bool asym = (w.asym_mask >> c1) & 1;
+#pragma unroll
+            for (int j = 0; j < CPT; j++) {
+                const int i = tid + j * T;
+                if (i >= PL) break;
+                const int y = i / W, x = i % W;
                 const float u = sgf[2 * (kk * Cfg::PLANE + y * Wh) + x] / (float)S::N;
-                sus[c * PL + i] += u;
                 float g;
                 if (lut) {
                     int iu = (int)(u * 255.0f);
@@ -317,21 +332,31 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
                     const float q = (u - kp.m) / kp.s;
                     g = expf(-(q * q) / 2.0f);
                 }
-                if ((w.asym_mask >> c) & 1) sh[c * PL + i] += (g - A[(long long)c * S::N + (long long)z * PL + i]) * kp.h;
-                else sh[c * PL + i] += g * kp.h;
+#pragma unroll
+                for (int c = 0; c < LHD_MAX_C; c++) {
+                    if (c != c1) continue;
+                    us[j][c] += u;
+                    if (asym) h[j][c] += (g - av[j][c]) * kp.h;
+                    else h[j][c] += g * kp.h;
+                }
             }
         }
     }
-    __syncthreads();
-    float *Hg = a.pool + w.Hg, *B = a.pool + w.Bk, *US = a.pool + w.USUM;
-    for (int i = tid; i < C * PL; i += T) {
-        const int c = i / PL, j = i % PL;
-        const long long g = (long long)c * S::N + (long long)z * PL + j;
-        const float h = sh[i];
-        Hg[g] = h;
-        if (w.keep_usum) US[g] = sus[i];
-        if (w.growth_type == 1) {
-            B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(A[g], h, w.dt) : clamp_ref(A[g] + h * w.dt, 0.0f, 1.0f);
+    float *Hg = a.pool + w.Hg + (long long)z * PL, *B = a.pool + w.Bk + (long long)z * PL;
+    float *US = a.pool + w.USUM + (long long)z * PL;
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const int i = tid + j * T;
+        if (i >= PL) break;
+#pragma unroll
+        for (int c = 0; c < LHD_MAX_C; c++) {
+            if (c >= C) break;
+            const long long g = (long long)c * S::N + i;
+            const float hv = h[j][c];
+            Hg[g] = hv;
+            if (w.keep_usum) US[g] = us[j][c];
+            if (w.growth_type == 1)
+                B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(av[j][c], hv, w.dt) : clamp_ref(av[j][c] + hv * w.dt, 0.0f, 1.0f);
         }
     }
 }
@@ -413,8 +438,30 @@ __device__ __forceinline__ float field_at(const FieldRef &f, int x, int y, int z
     return f.p[x + W * (y + H * z)];
 }
 
-template <int W, int H, int D, bool PERM>
-__device__ __forceinline__ void sobel3(const FieldRef &f, int x, int y, int z, int kind, int faces, float (&g)[3]) {
+// ------------------------------------------------------------------------------------
+// Tiled variants of local and transport.  A workgroup owns a TX x TY x TZ block of
+// cells (whole x rows) and stages the block plus a one-cell halo in LDS, with every
+// halo value resolved once at load time, so the per-cell loops carry no branches.
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+struct TileCfg {
+    static constexpr int TX = W;
+    static constexpr int TY = H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1);
+    static constexpr int TZ = D % 4 == 0 ? 4 : (D % 2 == 0 ? 2 : 1);
+    static constexpr int SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
+    static constexpr int NS = SX * SY * SZ;   // staged slots
+    static constexpr int NT = TX * TY * TZ;   // cells per tile
+    static constexpr int TILES = (H / TY) * (D / TZ);
+    static constexpr int THREADS = 256;
+    static constexpr int CPT = (NT + THREADS - 1) / THREADS;
+    __device__ static __forceinline__ int slot(int tx, int ty, int tz) { return tx + SX * (ty + SY * tz); }
+};
+
+// Sobel from a staged tile; taps in the reference order dz, dy, dx.  Tap weights are
+// +-1, +-2, +-4, so each product is exact and fma(v, c, g) rounds exactly like the
+// reference's separate multiply and add.
+template <int SX, int SY, bool PERM>
+__device__ __forceinline__ void sobel_lds(const float *sf, int tx, int ty, int tz, float (&g)[3]) {
     constexpr int SW[3] = {1, 2, 1};
     constexpr int SD[3] = {-1, 0, 1};
     g[0] = 0.0f; g[1] = 0.0f; g[2] = 0.0f;
@@ -429,149 +476,151 @@ __device__ __forceinline__ void sobel3(const FieldRef &f, int x, int y, int z, i
                 const int gz = SW[dx + 1] * SW[dy + 1] * SD[dz + 1];
                 const int c0 = PERM ? gz : gx, c1 = PERM ? gx : gy, c2 = PERM ? gy : gz;
                 if (c0 == 0 && c1 == 0 && c2 == 0) continue;
-                const float v = field_at<W, H, D>(f, x + dx, y + dy, z + dz, kind, faces);
-                if (c0 != 0) g[0] = g[0] + v * (float)c0;
-                if (c1 != 0) g[1] = g[1] + v * (float)c1;
-                if (c2 != 0) g[2] = g[2] + v * (float)c2;
+                const float v = sf[(tx + dx + 1) + SX * ((ty + dy + 1) + SY * (tz + dz + 1))];
+                if (c0 != 0) g[0] = __builtin_fmaf(v, (float)c0, g[0]);
+                if (c1 != 0) g[1] = __builtin_fmaf(v, (float)c1, g[1]);
+                if (c2 != 0) g[2] = __builtin_fmaf(v, (float)c2, g[2]);
             }
 }
 
-template <int W, int H, int D>
-__device__ __forceinline__ void cdiff3(const FieldRef &f, int x, int y, int z, int kind, int faces, float (&g)[3]) {
-    g[0] = (field_at<W, H, D>(f, x + 1, y, z, kind, faces) - field_at<W, H, D>(f, x - 1, y, z, kind, faces)) * 0.5f;
-    g[1] = (field_at<W, H, D>(f, x, y + 1, z, kind, faces) - field_at<W, H, D>(f, x, y - 1, z, kind, faces)) * 0.5f;
-    g[2] = (field_at<W, H, D>(f, x, y, z + 1, kind, faces) - field_at<W, H, D>(f, x, y, z - 1, kind, faces)) * 0.5f;
+template <int SX, int SY>
+__device__ __forceinline__ void cdiff_lds(const float *sf, int tx, int ty, int tz, float (&g)[3]) {
+    auto at = [&](int x, int y, int z) { return sf[(x + 1) + SX * ((y + 1) + SY * (z + 1))]; };
+    g[0] = (at(tx + 1, ty, tz) - at(tx - 1, ty, tz)) * 0.5f;
+    g[1] = (at(tx, ty + 1, tz) - at(tx, ty - 1, tz)) * 0.5f;
+    g[2] = (at(tx, ty, tz + 1) - at(tx, ty, tz - 1)) * 0.5f;
 }
 
-template <int W, int H, int D>
-__device__ __forceinline__ void gradient(const DevWorld &w, const FieldRef &f, int x, int y, int z, float (&g)[3]) {
-    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+template <int SX, int SY>
+__device__ __forceinline__ void gradient_lds(const DevWorld &w, const float *sf, int tx, int ty, int tz, float (&g)[3]) {
     if (w.gradient == 1) {
-        if (w.quirks & LHQ_SOBEL_PERMUTE) sobel3<W, H, D, true>(f, x, y, z, w.sobel_border, faces, g);
-        else sobel3<W, H, D, false>(f, x, y, z, w.sobel_border, faces, g);
+        if (w.quirks & LHQ_SOBEL_PERMUTE) sobel_lds<SX, SY, true>(sf, tx, ty, tz, g);
+        else sobel_lds<SX, SY, false>(sf, tx, ty, tz, g);
     } else {
-        cdiff3<W, H, D>(f, x, y, z, w.sobel_border, faces, g);
+        cdiff_lds<SX, SY>(sf, tx, ty, tz, g);
     }
 }
 
 template <int W, int H, int D>
-__global__ __launch_bounds__(256) void k_local(PhaseArgs a, int) {
+__global__ __launch_bounds__(256) void k_local_tiled(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
-    constexpr int BPW = (S::N + 255) / 256;
-    const int wi = blockIdx.x / BPW;
-    const int i = (blockIdx.x % BPW) * 256 + threadIdx.x;
-    if (i >= S::N) return;
+    using T = TileCfg<W, H, D>;
+    __shared__ float sf[(LHD_MAX_C + 1) * T::NS];
+    const int wi = blockIdx.x / T::TILES, tile = blockIdx.x % T::TILES;
+    const int ty0 = (tile % (H / T::TY)) * T::TY, tz0 = (tile / (H / T::TY)) * T::TZ;
     const DevWorld &w = a.worlds[a.ids[wi]];
-    const int x = i % W, y = (i / W) % H, z = i / (W * H);
-    const int C = w.C;
-    const float *A = A_cur(a, w);
+    const int tid = threadIdx.x, C = w.C;
     const long long BIG = 0x7FFFFFFFFFFFFFFFLL;
-    // gradient sources: Sobel reads SUM and H; central differences read SUM_S and U_S
-    FieldRef fs;
-    const bool cd_copy = w.gradient == 0 && !w.blur;
-    if (w.gradient == 1 || !w.blur) fs = FieldRef{a.pool + w.S, BIG};
-    else fs = FieldRef{a.pool + w.SUMS, BIG};
-    float na[3];
-    gradient<W, H, D>(w, fs, x, y, z, na);
+    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+    // stage S (or SUM_S) and H_c (or U_S_c) with their halos
+    for (int f = 0; f <= C; f++) {
+        FieldRef fr;
+        if (f == 0) fr = FieldRef{(w.gradient == 0 && w.blur) ? a.pool + w.SUMS : a.pool + w.S, BIG};
+        else {
+            const int c = f - 1;
+            if (w.gradient == 1) fr = FieldRef{a.pool + w.Hg + (long long)c * S::N, BIG};
+            else if (w.blur) fr = FieldRef{a.pool + w.US + (long long)c * S::N, BIG};
+            else fr = FieldRef{a.pool + w.Hg + (long long)c * S::N,
+                               (w.quirks & LHQ_CD_COPY) ? (5 * S::P / 2) - (long long)c * S::P : BIG};
+        }
+        for (int q = tid; q < T::NS; q += T::THREADS) {
+            const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
+            sf[f * T::NS + q] = field_at<W, H, D>(fr, sx - 1, ty0 + sy - 1, tz0 + sz - 1, w.sobel_border, faces);
+        }
+    }
+    __syncthreads();
+    const float *A = A_cur(a, w);
     const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
-    const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
     const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
     const bool wall = w.border == 1;
     const bool qa = (w.quirks & LHQ_PREALPHA_INDEX) != 0;
-    for (int c = 0; c < C; c++) {
-        FieldRef fh;
-        if (w.gradient == 1) fh = FieldRef{a.pool + w.Hg + (long long)c * S::N, BIG};
-        else if (w.blur) fh = FieldRef{a.pool + w.US + (long long)c * S::N, BIG};
-        else {
-            // U_S <- H by two overlapping copies covering the first 5P/2 padded floats
-            const long long cut = (w.quirks & LHQ_CD_COPY) ? (5 * S::P / 2) - (long long)c * S::P : BIG;
-            fh = FieldRef{a.pool + w.Hg + (long long)c * S::N, cut};
-        }
-        (void)cd_copy;
-        float nu[3];
-        gradient<W, H, D>(w, fh, x, y, z, nu);
-        // alpha (lenia.cpp:647-672, read lenia.cpp:708)
-        float al;
-        {
-            long long r = (long long)c * S::N + i;
-            int cs = c;
-            long long is = i;
-            bool ok = true;
-            if (qa) { cs = (int)(r / S::P); is = r - (long long)cs * S::P; ok = cs < C && is < S::N; }
-            if (ok) {
-                float pg = A[(long long)cs * S::N + is] / w.theta_a;
-                pg = pg * pg;
-                al = clamp_ref(pg, 0.0f, 1.0f);
+#pragma unroll
+    for (int j = 0; j < T::CPT; j++) {
+        const int cell = tid + j * T::THREADS;
+        if (cell >= T::NT) break;
+        const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
+        const int x = tx, y = ty0 + ty, z = tz0 + tz;
+        const long long i = x + (long long)W * (y + (long long)H * z);
+        float na[3];
+        gradient_lds<T::SX, T::SY>(w, sf, tx, ty, tz, na);
+        const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+        for (int c = 0; c < C; c++) {
+            float nu[3];
+            gradient_lds<T::SX, T::SY>(w, sf + (c + 1) * T::NS, tx, ty, tz, nu);
+            float al;
+            {
+                const long long r = (long long)c * S::N + i;
+                int cs = c;
+                long long is = i;
+                bool ok = true;
+                if (qa) { cs = (int)(r / S::P); is = r - (long long)cs * S::P; ok = cs < C && is < S::N; }
+                if (ok) {
+                    float pg = A[(long long)cs * S::N + is] / w.theta_a;
+                    pg = pg * pg;
+                    al = clamp_ref(pg, 0.0f, 1.0f);
+                } else {
+                    al = 0.0f;
+                }
+            }
+            const float al1 = 1.0f - al;
+            float F[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
+            float *V = a.pool + w.V + (long long)c * 3 * S::N;
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            if (w.v_live) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
+            }
+            if (w.mu_vel) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
+            }
+            float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
+            if (w.mode == 2) {
+                float vn[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const float src = w.adv_vel ? vn[d] : F[d];
+                    MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
+                }
             } else {
-                al = 0.0f;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    float m = (w.mu_vel ? v[d] : F[d]) * dt;
+                    m = fmax_ref(m, nma);
+                    m = fmin_ref(m, ma);
+                    m = pos[d] + m;
+                    if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
+                    MU[(long long)d * S::N + i] = m;
+                }
             }
-        }
-        const float al1 = 1.0f - al;
-        float F[3];
+            if (!w.mu_vel) {
 #pragma unroll
-        for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
-        // mu and velocity (reintegration.cpp:321-602)
-        float *V = a.pool + w.V + (long long)c * 3 * S::N;
-        float v[3] = {0.0f, 0.0f, 0.0f};
-        if (w.v_live) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
-        }
-        if (w.mu_vel) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
-        }
-        float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
-        if (w.mode == 2) {
-            // advection displacement clamp(V*dt or F*dt, -ma, ma) (reintegration.cpp:1142-1166)
-            float vn[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const float src = w.adv_vel ? vn[d] : F[d];
-                MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
+                for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
             }
-        } else {
+            if (w.v_live) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                float m = (w.mu_vel ? v[d] : F[d]) * dt;
-                m = fmax_ref(m, nma);
-                m = fmin_ref(m, ma);
-                m = pos[d] + m;
-                if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
-                MU[(long long)d * S::N + i] = m;
+                for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
             }
-        }
-        if (!w.mu_vel) {
+            if (w.need_f) {
+                float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
 #pragma unroll
-            for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
-        }
-        if (w.v_live) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
-        }
-        if (w.need_f) {
-            float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
-#pragma unroll
-            for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
+                for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
+            }
         }
     }
 }
 
-// ------------------------------------------------------------------------------------
-// transport: per destination cell.  SRT / RT gather (27 sources for dd=1) or trilinear
-// advection into TT, then the end-of-step TRM: clamp, A <- TT, GA, SUM.
-// ------------------------------------------------------------------------------------
+// Per-cell transport for the modes the tiled RT path does not cover (SRT, ADV, RT with
+// dd > 1); the reference gather of rt_border_* / srt / adv_* for one cell.
 template <int W, int H, int D>
-__global__ __launch_bounds__(256) void k_transport(PhaseArgs a, int) {
+__device__ __forceinline__ void transport_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
+                                               float (&tt)[LHD_MAX_C]) {
     using S = Shape<W, H, D>;
-    constexpr int BPW = (S::N + 255) / 256;
-    const int wi = blockIdx.x / BPW;
-    const int i = (blockIdx.x % BPW) * 256 + threadIdx.x;
-    if (i >= S::N) return;
-    const DevWorld &w = a.worlds[a.ids[wi]];
-    const int x = i % W, y = (i / W) % H, z = i / (W * H);
+    const long long i = x + (long long)W * (y + (long long)H * z);
     const int C = w.C;
     const float *Acur = A_cur(a, w);
     const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? Acur : a.pool + w.Bk;
@@ -580,7 +629,6 @@ __global__ __launch_bounds__(256) void k_transport(PhaseArgs a, int) {
     const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
     const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
     const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
-    float tt[LHD_MAX_C] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (w.mode == 0) {
 #pragma unroll
         for (int c = 0; c < LHD_MAX_C; c++) {
@@ -672,8 +720,15 @@ __global__ __launch_bounds__(256) void k_transport(PhaseArgs a, int) {
             tt[c] = acc;
         }
     }
-    // TRM (lenia.cpp:930-1035)
+}
+
+// End-of-step TRM for one cell (lenia.cpp:930-1035).
+template <int W, int H, int D>
+__device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, long long i, const float (&tt)[LHD_MAX_C]) {
+    using S = Shape<W, H, D>;
+    const int C = w.C;
     const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
+    const float *Acur = A_cur(a, w);
     float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
     const float *Hg = a.pool + w.Hg;
     float ttsum = 0.0f;
@@ -696,6 +751,112 @@ __global__ __launch_bounds__(256) void k_transport(PhaseArgs a, int) {
     }
     const float sold = (w.quirks & LHQ_SUM_RESET) ? 0.0f : Sg[i];
     Sg[i] = w.stam * sold + ttsum * w.stas;
+}
+
+// RT with dd = 1 from LDS: each staged source carries its nine per-axis overlap weights
+// (toward the destinations at offsets -1, 0, +1 along x, y and z), computed once with
+// exactly the expression the gather would evaluate for that (destination, source)
+// pair, so each destination's 27 products and their summation order are unchanged.
+template <int W, int H, int D>
+__global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    using T = TileCfg<W, H, D>;
+    __shared__ float s_arr[T::NS];
+    __shared__ float s_w[9 * T::NS];
+    const int wi = blockIdx.x / T::TILES, tile = blockIdx.x % T::TILES;
+    const int ty0 = (tile % (H / T::TY)) * T::TY, tz0 = (tile / (H / T::TY)) * T::TZ;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int tid = threadIdx.x, C = w.C;
+    float tt[T::CPT][LHD_MAX_C];
+#pragma unroll
+    for (int j = 0; j < T::CPT; j++)
+#pragma unroll
+        for (int c = 0; c < LHD_MAX_C; c++) tt[j][c] = 0.0f;
+    if (w.mode == 1 && w.dd == 1) {
+        const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A_cur(a, w) : a.pool + w.Bk;
+        const float *MU = a.pool + w.MU;
+        const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+        const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
+        const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
+        const bool torus = w.border == 0;
+        const float ext[3] = {(float)W, (float)H, (float)D};
+        const float half[3] = {(float)(W / 2), (float)(H / 2), (float)(D / 2)};
+        const float si = w.si, clip = w.clip, sn = w.sn;
+#pragma unroll
+        for (int c = 0; c < LHD_MAX_C; c++) {
+            if (c >= nrt) break;
+            __syncthreads();
+            for (int q = tid; q < T::NS; q += T::THREADS) {
+                const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
+                const int g[3] = {sx - 1, ty0 + sy - 1, tz0 + sz - 1};
+                int qx = g[0], qy = g[1], qz = g[2];
+                float arr = 0.0f, wv[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) wv[k] = 0.0f;
+                if (halo_src<W, H, D>(qx, qy, qz, tkind, faces)) {
+                    const long long s = qx + W * (qy + (long long)H * qz);
+                    arr = ARR[(long long)c * S::N + s];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const float mu = MU[((long long)c * 3 + d) * S::N + s];
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const float pos = (float)(g[d] - (k - 1)) + 0.5f;
+                            float dv = pos - mu;
+                            if (torus) {
+                                if (dv > half[d]) dv = dv - ext[d];
+                                if (dv < -half[d]) dv = dv + ext[d];
+                            }
+                            dv = dv < 0.0f ? -dv : dv;
+                            dv = si - dv;
+                            dv = fmax_ref(dv, 0.0f);
+                            wv[d * 3 + k] = fmin_ref(dv, clip);
+                        }
+                    }
+                }
+                s_arr[q] = arr;
+#pragma unroll
+                for (int k = 0; k < 9; k++) s_w[k * T::NS + q] = wv[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < T::CPT; j++) {
+                const int cell = tid + j * T::THREADS;
+                if (cell >= T::NT) break;
+                const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
+                float acc = tt[j][c];
+#pragma unroll
+                for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const int q = T::slot(tx + dx + 1, ty + dy + 1, tz + dz + 1);
+                            float wt = s_arr[q] * s_w[(dx + 1) * T::NS + q];
+                            wt = wt * s_w[(3 + dy + 1) * T::NS + q];
+                            wt = wt * s_w[(6 + dz + 1) * T::NS + q];
+                            wt = wt * sn;
+                            acc = acc + wt;
+                        }
+                tt[j][c] = acc;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < T::CPT; j++) {
+            const int cell = tid + j * T::THREADS;
+            if (cell >= T::NT) break;
+            const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
+            transport_cell<W, H, D>(a, w, tx, ty0 + ty, tz0 + tz, tt[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < T::CPT; j++) {
+        const int cell = tid + j * T::THREADS;
+        if (cell >= T::NT) break;
+        const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
+        trm_cell<W, H, D>(a, w, tx + (long long)W * ((ty0 + ty) + (long long)H * (tz0 + tz)), tt[j]);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -723,12 +884,12 @@ struct Launch {
         hipLaunchKernelGGL((k_inv_blur<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
     }
     static void local(const PhaseArgs &a, int, void *st) {
-        constexpr int BPW = (W * H * D + 255) / 256;
-        hipLaunchKernelGGL((k_local<W, H, D>), dim3(a.nworlds * BPW), dim3(256), 0, (hipStream_t)st, a, 0);
+        hipLaunchKernelGGL((k_local_tiled<W, H, D>), dim3(a.nworlds * TileCfg<W, H, D>::TILES), dim3(256), 0,
+                           (hipStream_t)st, a, 0);
     }
     static void transport(const PhaseArgs &a, int, void *st) {
-        constexpr int BPW = (W * H * D + 255) / 256;
-        hipLaunchKernelGGL((k_transport<W, H, D>), dim3(a.nworlds * BPW), dim3(256), 0, (hipStream_t)st, a, 0);
+        hipLaunchKernelGGL((k_transport_tiled<W, H, D>), dim3(a.nworlds * TileCfg<W, H, D>::TILES), dim3(256), 0,
+                           (hipStream_t)st, a, 0);
     }
     static constexpr ShapeOps ops() {
         return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z};
@@ -803,6 +964,15 @@ __global__ __launch_bounds__(256) void k_mc_case(const float *f, int W, int H, i
         if (val > iso) v |= 1 << k;
     }
     out[i] = (unsigned char)v;
+}
+
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
+}
+
+void lh_launch_copy(const void *src, void *dst, size_t bytes, void *st) {
+    const long long n = (long long)(bytes / 16);
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)st, (const float4 *)src, (float4 *)dst, n);
 }
 
 void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st) {

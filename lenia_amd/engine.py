@@ -114,6 +114,9 @@ class Engine:
             out[name.value.decode()] = (n.value, ms.value)
         return out
 
+    def calibration_copy(self, nbytes: int):
+        L.check(self._L.lh_calibration_copy(self._h, int(nbytes)), "lh_calibration_copy")
+
     def profile_reset(self):
         L.check(self._L.lh_profile_reset(self._h), "lh_profile_reset")
 
