@@ -54,9 +54,28 @@ def build(verbose: bool = True) -> str:
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_shim()
     if verbose:
         print(f"built {LIB} in {time.time() - t0:.1f}s")
     return LIB
+
+
+SHIM = os.path.join(HERE, "liblenia_ref_shim.so")
+HARNESS = os.path.join(HERE, "ref_harness")
+
+
+def build_shim() -> None:
+    """The reference-signature shim (host C++ over the C-ABI) and its 2-thread harness."""
+    inc = ["-I", os.path.join(REPO, "include")]
+    common = ["-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-Wall"] + inc
+    subprocess.run(["g++"] + common + ["-shared", "-o", SHIM + ".tmp",
+                                       os.path.join(HERE, "shim", "lenia_ref_shim.cpp"),
+                                       "-L", HERE, "-llenia_hip", "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(SHIM + ".tmp", SHIM)
+    subprocess.run(["g++"] + common + ["-o", HARNESS + ".tmp", os.path.join(HERE, "shim", "ref_harness.cpp"),
+                                       "-L", HERE, "-llenia_ref_shim", "-llenia_hip", "-lpthread",
+                                       "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(HARNESS + ".tmp", HARNESS)
 
 
 if __name__ == "__main__":
