@@ -88,7 +88,6 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
     using S = Shape<W, H, D>;
     using Cfg = FwdCfg<W, H, D>;
     constexpr int NP = Cfg::NP, Wh = S::Wh;
-    __shared__ float sreal[NP * W * H];
     __shared__ float2 sspec[NP * H * Wh];
     const int src = arg & 0xFF, nfmax = arg >> 8;
     const int groups = D / NP;
@@ -115,14 +114,20 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
     }
     if (f >= nf) return;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NP * W * H; i += Cfg::THREADS) sreal[i] = in[(long long)z0 * S::PL + i];
-    __syncthreads();
+    // x: rows y, y+1 straight from HBM (W is a multiple of 4: 16-byte row starts)
     for (int t = tid; t < NP * (H / 2); t += Cfg::THREADS) {
         const int p = t / (H / 2), y = 2 * (t % (H / 2));
+        const float4 *r0 = reinterpret_cast<const float4 *>(in + (long long)(z0 + p) * S::PL + y * W);
+        const float4 *r1 = r0 + W / 4;
         float2 line[W];
-        const float *r0 = sreal + p * S::PL + y * W;
 #pragma unroll
-        for (int x = 0; x < W; x++) line[x] = make_float2(r0[x], r0[x + W]);
+        for (int x = 0; x < W / 4; x++) {
+            const float4 u = r0[x], v = r1[x];
+            line[4 * x + 0] = make_float2(u.x, v.x);
+            line[4 * x + 1] = make_float2(u.y, v.y);
+            line[4 * x + 2] = make_float2(u.z, v.z);
+            line[4 * x + 3] = make_float2(u.w, v.w);
+        }
         fft_line<W, false>(line, a.twW);
         float2 *o0 = sspec + (p * H + y) * Wh, *o1 = o0 + Wh;
 #pragma unroll
@@ -425,19 +430,6 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
 // local: per cell.  Gradients of S and H_c (Sobel with the reference's axis order, or
 // central differences), alpha (with the PREALPHA index quirk), F, mu, V.
 // ------------------------------------------------------------------------------------
-struct FieldRef {
-    const float *p;   // interior values [N]
-    long long cut;    // padded flat index bound: interior cells at or above read 0
-};
-
-template <int W, int H, int D>
-__device__ __forceinline__ float field_at(const FieldRef &f, int x, int y, int z, int kind, int faces) {
-    using S = Shape<W, H, D>;
-    if (!halo_src<W, H, D>(x, y, z, kind, faces)) return 0.0f;
-    if (S::pidx(x, y, z) >= f.cut) return 0.0f;
-    return f.p[x + W * (y + H * z)];
-}
-
 // ------------------------------------------------------------------------------------
 // Tiled variants of local and transport.  A workgroup owns a TX x TY x TZ block of
 // cells (whole x rows) and stages the block plus a one-cell halo in LDS, with every
@@ -510,22 +502,27 @@ __global__ __launch_bounds__(256) void k_local_tiled(PhaseArgs a, int) {
     const int ty0 = (tile % (H / T::TY)) * T::TY, tz0 = (tile / (H / T::TY)) * T::TZ;
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C;
-    const long long BIG = 0x7FFFFFFFFFFFFFFFLL;
     const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
-    // stage S (or SUM_S) and H_c (or U_S_c) with their halos
-    for (int f = 0; f <= C; f++) {
-        FieldRef fr;
-        if (f == 0) fr = FieldRef{(w.gradient == 0 && w.blur) ? a.pool + w.SUMS : a.pool + w.S, BIG};
-        else {
-            const int c = f - 1;
-            if (w.gradient == 1) fr = FieldRef{a.pool + w.Hg + (long long)c * S::N, BIG};
-            else if (w.blur) fr = FieldRef{a.pool + w.US + (long long)c * S::N, BIG};
-            else fr = FieldRef{a.pool + w.Hg + (long long)c * S::N,
-                               (w.quirks & LHQ_CD_COPY) ? (5 * S::P / 2) - (long long)c * S::P : BIG};
-        }
-        for (int q = tid; q < T::NS; q += T::THREADS) {
-            const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
-            sf[f * T::NS + q] = field_at<W, H, D>(fr, sx - 1, ty0 + sy - 1, tz0 + sz - 1, w.sobel_border, faces);
+    // Stage S (or SUM_S) and H_c (or U_S_c) with their halos.  Each slot's halo source
+    // is resolved once and shared by all fields.  The non-blur central-difference copy
+    // (reintegration.cpp:179-186) leaves U_S zero from padded index 5P/2 on.
+    const bool cdcopy = w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
+    const float *fbase[LHD_MAX_C + 1];
+    fbase[0] = (w.gradient == 0 && w.blur) ? a.pool + w.SUMS : a.pool + w.S;
+#pragma unroll
+    for (int c = 0; c < LHD_MAX_C; c++)
+        fbase[c + 1] = ((w.gradient == 0 && w.blur) ? a.pool + w.US : a.pool + w.Hg) + c * S::N;
+    for (int q = tid; q < T::NS; q += T::THREADS) {
+        const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
+        int x = sx - 1, y = ty0 + sy - 1, z = tz0 + sz - 1;
+        const bool ok = halo_src<W, H, D>(x, y, z, w.sobel_border, faces);
+        const int lin = x + W * (y + H * z);
+        const int pq = (int)S::pidx(x, y, z);
+#pragma unroll
+        for (int f = 0; f <= LHD_MAX_C; f++) {
+            if (f > C) break;
+            const bool cut = cdcopy && f > 0 && (long long)pq + (long long)(f - 1) * S::P >= 5 * S::P / 2;
+            sf[f * T::NS + q] = (ok && !cut) ? fbase[f][lin] : 0.0f;
         }
     }
     __syncthreads();
@@ -549,13 +546,13 @@ __global__ __launch_bounds__(256) void k_local_tiled(PhaseArgs a, int) {
             gradient_lds<T::SX, T::SY>(w, sf + (c + 1) * T::NS, tx, ty, tz, nu);
             float al;
             {
-                const long long r = (long long)c * S::N + i;
-                int cs = c;
-                long long is = i;
+                // PREALPHA written at c*P+i and read at c*N+i (lenia.cpp:660, 708)
+                const int r = c * S::N + (int)i;
+                int cs = c, is = (int)i;
                 bool ok = true;
-                if (qa) { cs = (int)(r / S::P); is = r - (long long)cs * S::P; ok = cs < C && is < S::N; }
+                if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = cs < C && is < S::N; }
                 if (ok) {
-                    float pg = A[(long long)cs * S::N + is] / w.theta_a;
+                    float pg = A[cs * S::N + is] / w.theta_a;
                     pg = pg * pg;
                     al = clamp_ref(pg, 0.0f, 1.0f);
                 } else {
@@ -757,10 +754,13 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
 // (toward the destinations at offsets -1, 0, +1 along x, y and z), computed once with
 // exactly the expression the gather would evaluate for that (destination, source)
 // pair, so each destination's 27 products and their summation order are unchanged.
+// A slot's halo source is resolved once and all its global loads (mu, source mass for
+// every channel) are issued before any weight is formed.
 template <int W, int H, int D>
 __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using T = TileCfg<W, H, D>;
+    constexpr int SPT = (T::NS + T::THREADS - 1) / T::THREADS;  // staged slots per thread
     __shared__ float s_arr[T::NS];
     __shared__ float s_w[9 * T::NS];
     const int wi = blockIdx.x / T::TILES, tile = blockIdx.x % T::TILES;
@@ -778,45 +778,60 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
         const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
         const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
         const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
+        // rt_border_torus wraps the displacement by +-extent past half the extent;
+        // rt_border_other does not (an infinite threshold never triggers)
         const bool torus = w.border == 0;
         const float ext[3] = {(float)W, (float)H, (float)D};
-        const float half[3] = {(float)(W / 2), (float)(H / 2), (float)(D / 2)};
+        const float half[3] = {torus ? (float)(W / 2) : __builtin_inff(), torus ? (float)(H / 2) : __builtin_inff(),
+                               torus ? (float)(D / 2) : __builtin_inff()};
         const float si = w.si, clip = w.clip, sn = w.sn;
+        // gather every staged slot's inputs for all channels up front
+        float mu_r[SPT][LHD_MAX_C][3], arr_r[SPT][LHD_MAX_C];
+        int gpos[SPT][3];
+#pragma unroll
+        for (int j = 0; j < SPT; j++) {
+            const int q = tid + j * T::THREADS;
+            const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
+            gpos[j][0] = sx - 1;
+            gpos[j][1] = ty0 + sy - 1;
+            gpos[j][2] = tz0 + sz - 1;
+            int x = gpos[j][0], y = gpos[j][1], z = gpos[j][2];
+            const bool ok = q < T::NS && halo_src<W, H, D>(x, y, z, tkind, faces);
+            const int s = x + W * (y + H * z);
+#pragma unroll
+            for (int c = 0; c < LHD_MAX_C; c++) {
+                arr_r[j][c] = (ok && c < nrt) ? ARR[c * S::N + s] : 0.0f;
+#pragma unroll
+                for (int d = 0; d < 3; d++) mu_r[j][c][d] = (ok && c < nrt) ? MU[(c * 3 + d) * S::N + s] : 0.0f;
+            }
+        }
 #pragma unroll
         for (int c = 0; c < LHD_MAX_C; c++) {
             if (c >= nrt) break;
             __syncthreads();
-            for (int q = tid; q < T::NS; q += T::THREADS) {
-                const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
-                const int g[3] = {sx - 1, ty0 + sy - 1, tz0 + sz - 1};
-                int qx = g[0], qy = g[1], qz = g[2];
-                float arr = 0.0f, wv[9];
 #pragma unroll
-                for (int k = 0; k < 9; k++) wv[k] = 0.0f;
-                if (halo_src<W, H, D>(qx, qy, qz, tkind, faces)) {
-                    const long long s = qx + W * (qy + (long long)H * qz);
-                    arr = ARR[(long long)c * S::N + s];
+            for (int j = 0; j < SPT; j++) {
+                const int q = tid + j * T::THREADS;
+                if (q >= T::NS) break;
+                s_arr[q] = arr_r[j][c];
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const float mu = MU[((long long)c * 3 + d) * S::N + s];
+                for (int d = 0; d < 3; d++) {
+                    const float mu = mu_r[j][c][d];
 #pragma unroll
-                        for (int k = 0; k < 3; k++) {
-                            const float pos = (float)(g[d] - (k - 1)) + 0.5f;
-                            float dv = pos - mu;
-                            if (torus) {
-                                if (dv > half[d]) dv = dv - ext[d];
-                                if (dv < -half[d]) dv = dv + ext[d];
-                            }
-                            dv = dv < 0.0f ? -dv : dv;
-                            dv = si - dv;
-                            dv = fmax_ref(dv, 0.0f);
-                            wv[d * 3 + k] = fmin_ref(dv, clip);
-                        }
+                    for (int k = 0; k < 3; k++) {
+                        const float pos = (float)(gpos[j][d] - (k - 1)) + 0.5f;
+                        float dv = pos - mu;
+                        // the two one-sided wraps of rt_border_torus (reintegration.cpp:733-755):
+                        // after dv -= ext for dv > half, dv > -half, so at most one fires and
+                        // dv - (+-ext) equals the reference's dv - ext / dv + ext exactly
+                        const float wr = dv - __builtin_copysignf(ext[d], dv);
+                        dv = __builtin_fabsf(dv) > half[d] ? wr : dv;
+                        // abs_ps clears the sign bit; si - |dv| is never -0 or NaN here, so
+                        // v_max / v_min give the same bits as _mm_max_ps / _mm_min_ps
+                        dv = si - __builtin_fabsf(dv);
+                        s_w[(d * 3 + k) * T::NS + q] = __builtin_fminf(__builtin_fmaxf(dv, 0.0f), clip);
                     }
                 }
-                s_arr[q] = arr;
-#pragma unroll
-                for (int k = 0; k < 9; k++) s_w[k * T::NS + q] = wv[k];
             }
             __syncthreads();
 #pragma unroll
