@@ -106,3 +106,46 @@ template <int N, bool INV>
 __device__ __forceinline__ void fft_line(float2 (&x)[N], const float2 *__restrict__ tw) {
     fft_stages<N, INV, 0, 1>(x, tw);
 }
+
+// Block-cooperative Stockham over LINES lines of N points held in LDS (line l at
+// a[l * LDA + i]); b is scratch of the same layout.  Every thread of the block calls it;
+// each stage spreads the LINES * N/R butterflies over the block's NT threads and ends
+// with a barrier.  Per butterfly the arithmetic is fft_line's, so a line transformed
+// here matches a register-resident transform bit for bit.  The result is left in a.
+template <int N, bool INV, int LINES, int LDA, int NT, int S, int NS>
+__device__ __forceinline__ void fft_lds_stages(float2 *src, float2 *dst, const float2 *__restrict__ tw) {
+    if constexpr (S < fft_nst(N)) {
+        constexpr int R = fft_radix(N, S);
+        constexpr int NR = N / R;
+        constexpr int TS = N / (NS * R);
+        for (int t = threadIdx.x; t < LINES * NR; t += NT) {
+            const int l = t / NR, j = t % NR;
+            const int k = j % NS;
+            float2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = src[l * LDA + j + r * NR];
+            if (k != 0) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[r] = c_mul(v[r], twiddle<INV>(tw, r * k * TS));
+            }
+            fft_butterfly<R, N, INV>(v, tw);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[l * LDA + base + r * NS] = v[r];
+        }
+        __syncthreads();
+        fft_lds_stages<N, INV, LINES, LDA, NT, S + 1, NS * R>(dst, src, tw);
+    }
+}
+
+template <int N, bool INV, int LINES, int LDA, int NT>
+__device__ __forceinline__ void fft_lds_batch(float2 *a, float2 *b, const float2 *__restrict__ tw) {
+    fft_lds_stages<N, INV, LINES, LDA, NT, 0, 1>(a, b, tw);
+    if constexpr (fft_nst(N) & 1) {
+        for (int t = threadIdx.x; t < LINES * N; t += NT) {
+            const int l = t / N, i = t % N;
+            a[l * LDA + i] = b[l * LDA + i];
+        }
+        __syncthreads();
+    }
+}

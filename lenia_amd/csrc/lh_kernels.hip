@@ -18,59 +18,7 @@
 //
 // All arithmetic is fp32 in a fixed order (built with -ffp-contract=off): results are
 // bit-identical to the CPU restatement used as the checker.
-#include <hip/hip_runtime.h>
-#include "lh_common.hpp"
-#include "lh_fft.hpp"
-
-#define PADR 4  // reference PADDING (render.h:291), used by the index-quirk emulation
-
-__device__ __forceinline__ float fmax_ref(float a, float b) { return a > b ? a : b; }  // _mm_max_ps
-__device__ __forceinline__ float fmin_ref(float a, float b) { return a < b ? a : b; }  // _mm_min_ps
-__device__ __forceinline__ float clamp_ref(float a, float lo, float hi) { return fmin_ref(fmax_ref(a, lo), hi); }
-
-template <int W, int H, int D>
-struct Shape {
-    static constexpr int N = W * H * D;
-    static constexpr int PL = W * H;
-    static constexpr int Wh = W / 2 + 1, Hh = H / 2 + 1, Dh = D / 2 + 1;
-    static constexpr int Nh = Wh * H * D;
-    static constexpr int Noct = Wh * Hh * Dh;
-    static constexpr int PW = W + 2 * PADR, PH = H + 2 * PADR, PD = D + 2 * PADR;
-    static constexpr long long P = (long long)PW * PH * PD;
-    // reference padded index PIDX of an interior-or-pad coordinate
-    __device__ static __forceinline__ long long pidx(int x, int y, int z) {
-        return (long long)(x + PADR) + (long long)(y + PADR) * PW + (long long)(z + PADR) * PW * PH;
-    }
-};
-
-// Resolves a possibly-out-of-range coordinate to the interior cell whose value a
-// reference halo holds there.  kind: 0 torus, 1 wall (replicate), 2 zero.  With
-// faces != 0 only the six faces are filled (edges/corners stay 0), as in
-// transposition.cpp.  Returns false where the halo holds 0.
-template <int W, int H, int D>
-__device__ __forceinline__ bool halo_src(int &x, int &y, int &z, int kind, int faces) {
-    const int ox = (x < 0) | (x >= W), oy = (y < 0) | (y >= H), oz = (z < 0) | (z >= D);
-    const int m = ox + oy + oz;
-    if (m == 0) return true;
-    if (kind == 2 || (faces && m >= 2)) return false;
-    if (kind == 0) {
-        x = x < 0 ? x + W : (x >= W ? x - W : x);
-        y = y < 0 ? y + H : (y >= H ? y - H : y);
-        z = z < 0 ? z + D : (z >= D ? z - D : z);
-    } else {
-        x = x < 0 ? 0 : (x >= W ? W - 1 : x);
-        y = y < 0 ? 0 : (y >= H ? H - 1 : y);
-        z = z < 0 ? 0 : (z >= D ? D - 1 : z);
-    }
-    return true;
-}
-
-__device__ __forceinline__ const float *A_cur(const PhaseArgs &a, const DevWorld &w) {
-    return a.pool + (a.parity ? w.A2 : w.A);
-}
-__device__ __forceinline__ float *A_next(const PhaseArgs &a, const DevWorld &w) {
-    return a.pool + (a.parity ? w.A : w.A2);
-}
+#include "lh_device.hpp"
 
 // ------------------------------------------------------------------------------------
 // fwd_xy: r2c along x (rows y, y+1 packed as one complex line) and the y-FFT, for NP
@@ -237,18 +185,6 @@ struct InvCfg {
     static constexpr int THREADS = 256;
 };
 
-__device__ __forceinline__ float soft_growth(float g, float h, float dt) {
-    // soft clip lenia.cpp:549-571 with fastexp reintegration.h:16-37
-    const float a0 = 1.0f, a1 = 1.0f, a2 = 1.0f / 2, a3 = 1.0f / 2 / 3, a4 = 1.0f / 2 / 3 / 4;
-    const float a5 = 1.0f / 2 / 3 / 4 / 5, a6 = 1.0f / 2 / 3 / 4 / 5 / 6, a7 = 1.0f / 2 / 3 / 4 / 5 / 6 / 7;
-    float v = g + h * dt;
-    v = -4.0f * (v - 0.5f);
-    float r = a7 * v + a6;
-    r = r * v + a5; r = r * v + a4; r = r * v + a3; r = r * v + a2; r = r * v + a1; r = r * v + a0;
-    r = 1.0f + r;
-    r = 1.0f / r;
-    return clamp_ref(r, 0.0f, 1.0f);
-}
 
 template <int W, int H, int D>
 __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
@@ -427,454 +363,6 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
 }
 
 // ------------------------------------------------------------------------------------
-// local: per cell.  Gradients of S and H_c (Sobel with the reference's axis order, or
-// central differences), alpha (with the PREALPHA index quirk), F, mu, V.
-// ------------------------------------------------------------------------------------
-// ------------------------------------------------------------------------------------
-// Tiled variants of local and transport.  A workgroup owns a TX x TY x TZ block of
-// cells (whole x rows) and stages the block plus a one-cell halo in LDS, with every
-// halo value resolved once at load time, so the per-cell loops carry no branches.
-// ------------------------------------------------------------------------------------
-template <int W, int H, int D>
-struct TileCfg {
-    static constexpr int TX = W;
-    static constexpr int TY = H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1);
-    static constexpr int TZ = D % 4 == 0 ? 4 : (D % 2 == 0 ? 2 : 1);
-    static constexpr int SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
-    static constexpr int NS = SX * SY * SZ;   // staged slots
-    static constexpr int NT = TX * TY * TZ;   // cells per tile
-    static constexpr int TILES = (H / TY) * (D / TZ);
-    static constexpr int THREADS = 256;
-    static constexpr int CPT = (NT + THREADS - 1) / THREADS;
-    __device__ static __forceinline__ int slot(int tx, int ty, int tz) { return tx + SX * (ty + SY * tz); }
-};
-
-// Sobel from a staged tile; taps in the reference order dz, dy, dx.  Tap weights are
-// +-1, +-2, +-4, so each product is exact and fma(v, c, g) rounds exactly like the
-// reference's separate multiply and add.
-template <int SX, int SY, bool PERM>
-__device__ __forceinline__ void sobel_lds(const float *sf, int tx, int ty, int tz, float (&g)[3]) {
-    constexpr int SW[3] = {1, 2, 1};
-    constexpr int SD[3] = {-1, 0, 1};
-    g[0] = 0.0f; g[1] = 0.0f; g[2] = 0.0f;
-#pragma unroll
-    for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const int gx = SD[dx + 1] * SW[dy + 1] * SW[dz + 1];
-                const int gy = SW[dx + 1] * SD[dy + 1] * SW[dz + 1];
-                const int gz = SW[dx + 1] * SW[dy + 1] * SD[dz + 1];
-                const int c0 = PERM ? gz : gx, c1 = PERM ? gx : gy, c2 = PERM ? gy : gz;
-                if (c0 == 0 && c1 == 0 && c2 == 0) continue;
-                const float v = sf[(tx + dx + 1) + SX * ((ty + dy + 1) + SY * (tz + dz + 1))];
-                if (c0 != 0) g[0] = __builtin_fmaf(v, (float)c0, g[0]);
-                if (c1 != 0) g[1] = __builtin_fmaf(v, (float)c1, g[1]);
-                if (c2 != 0) g[2] = __builtin_fmaf(v, (float)c2, g[2]);
-            }
-}
-
-template <int SX, int SY>
-__device__ __forceinline__ void cdiff_lds(const float *sf, int tx, int ty, int tz, float (&g)[3]) {
-    auto at = [&](int x, int y, int z) { return sf[(x + 1) + SX * ((y + 1) + SY * (z + 1))]; };
-    g[0] = (at(tx + 1, ty, tz) - at(tx - 1, ty, tz)) * 0.5f;
-    g[1] = (at(tx, ty + 1, tz) - at(tx, ty - 1, tz)) * 0.5f;
-    g[2] = (at(tx, ty, tz + 1) - at(tx, ty, tz - 1)) * 0.5f;
-}
-
-template <int SX, int SY>
-__device__ __forceinline__ void gradient_lds(const DevWorld &w, const float *sf, int tx, int ty, int tz, float (&g)[3]) {
-    if (w.gradient == 1) {
-        if (w.quirks & LHQ_SOBEL_PERMUTE) sobel_lds<SX, SY, true>(sf, tx, ty, tz, g);
-        else sobel_lds<SX, SY, false>(sf, tx, ty, tz, g);
-    } else {
-        cdiff_lds<SX, SY>(sf, tx, ty, tz, g);
-    }
-}
-
-template <int W, int H, int D>
-__global__ __launch_bounds__(256) void k_local_tiled(PhaseArgs a, int) {
-    using S = Shape<W, H, D>;
-    using T = TileCfg<W, H, D>;
-    __shared__ float sf[(LHD_MAX_C + 1) * T::NS];
-    const int wi = blockIdx.x / T::TILES, tile = blockIdx.x % T::TILES;
-    const int ty0 = (tile % (H / T::TY)) * T::TY, tz0 = (tile / (H / T::TY)) * T::TZ;
-    const DevWorld &w = a.worlds[a.ids[wi]];
-    const int tid = threadIdx.x, C = w.C;
-    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
-    // Stage S (or SUM_S) and H_c (or U_S_c) with their halos.  Each slot's halo source
-    // is resolved once and shared by all fields.  The non-blur central-difference copy
-    // (reintegration.cpp:179-186) leaves U_S zero from padded index 5P/2 on.
-    const bool cdcopy = w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
-    const float *fbase[LHD_MAX_C + 1];
-    fbase[0] = (w.gradient == 0 && w.blur) ? a.pool + w.SUMS : a.pool + w.S;
-#pragma unroll
-    for (int c = 0; c < LHD_MAX_C; c++)
-        fbase[c + 1] = ((w.gradient == 0 && w.blur) ? a.pool + w.US : a.pool + w.Hg) + c * S::N;
-    for (int q = tid; q < T::NS; q += T::THREADS) {
-        const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
-        int x = sx - 1, y = ty0 + sy - 1, z = tz0 + sz - 1;
-        const bool ok = halo_src<W, H, D>(x, y, z, w.sobel_border, faces);
-        const int lin = x + W * (y + H * z);
-        const int pq = (int)S::pidx(x, y, z);
-#pragma unroll
-        for (int f = 0; f <= LHD_MAX_C; f++) {
-            if (f > C) break;
-            const bool cut = cdcopy && f > 0 && (long long)pq + (long long)(f - 1) * S::P >= 5 * S::P / 2;
-            sf[f * T::NS + q] = (ok && !cut) ? fbase[f][lin] : 0.0f;
-        }
-    }
-    __syncthreads();
-    const float *A = A_cur(a, w);
-    const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
-    const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
-    const bool wall = w.border == 1;
-    const bool qa = (w.quirks & LHQ_PREALPHA_INDEX) != 0;
-#pragma unroll
-    for (int j = 0; j < T::CPT; j++) {
-        const int cell = tid + j * T::THREADS;
-        if (cell >= T::NT) break;
-        const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
-        const int x = tx, y = ty0 + ty, z = tz0 + tz;
-        const long long i = x + (long long)W * (y + (long long)H * z);
-        float na[3];
-        gradient_lds<T::SX, T::SY>(w, sf, tx, ty, tz, na);
-        const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
-        for (int c = 0; c < C; c++) {
-            float nu[3];
-            gradient_lds<T::SX, T::SY>(w, sf + (c + 1) * T::NS, tx, ty, tz, nu);
-            float al;
-            {
-                // PREALPHA written at c*P+i and read at c*N+i (lenia.cpp:660, 708)
-                const int r = c * S::N + (int)i;
-                int cs = c, is = (int)i;
-                bool ok = true;
-                if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = cs < C && is < S::N; }
-                if (ok) {
-                    float pg = A[cs * S::N + is] / w.theta_a;
-                    pg = pg * pg;
-                    al = clamp_ref(pg, 0.0f, 1.0f);
-                } else {
-                    al = 0.0f;
-                }
-            }
-            const float al1 = 1.0f - al;
-            float F[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
-            float *V = a.pool + w.V + (long long)c * 3 * S::N;
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            if (w.v_live) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
-            }
-            if (w.mu_vel) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
-            }
-            float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
-            if (w.mode == 2) {
-                float vn[3];
-#pragma unroll
-                for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const float src = w.adv_vel ? vn[d] : F[d];
-                    MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
-                }
-            } else {
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    float m = (w.mu_vel ? v[d] : F[d]) * dt;
-                    m = fmax_ref(m, nma);
-                    m = fmin_ref(m, ma);
-                    m = pos[d] + m;
-                    if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
-                    MU[(long long)d * S::N + i] = m;
-                }
-            }
-            if (!w.mu_vel) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
-            }
-            if (w.v_live) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
-            }
-            if (w.need_f) {
-                float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
-#pragma unroll
-                for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
-            }
-        }
-    }
-}
-
-// Per-cell transport for the modes the tiled RT path does not cover (SRT, ADV, RT with
-// dd > 1); the reference gather of rt_border_* / srt / adv_* for one cell.
-template <int W, int H, int D>
-__device__ __forceinline__ void transport_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
-                                               float (&tt)[LHD_MAX_C]) {
-    using S = Shape<W, H, D>;
-    const long long i = x + (long long)W * (y + (long long)H * z);
-    const int C = w.C;
-    const float *Acur = A_cur(a, w);
-    const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? Acur : a.pool + w.Bk;
-    const float *MU = a.pool + w.MU;
-    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
-    const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
-    const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
-    const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
-    if (w.mode == 0) {
-#pragma unroll
-        for (int c = 0; c < LHD_MAX_C; c++) {
-            if (c >= nrt) break;
-            const float *mu = MU + (long long)c * 3 * S::N;
-            float m[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                float dv = pos[d] - mu[(long long)d * S::N + i];
-                dv = dv < 0.0f ? -dv : dv;
-                dv = w.si - dv;
-                m[d] = fmin_ref(fmax_ref(dv, 0.0f), w.clip);
-            }
-            float wt = ARR[(long long)c * S::N + i] * m[0];
-            wt = wt * m[1];
-            wt = wt * m[2];
-            wt = wt * w.sn;
-            tt[c] = tt[c] + wt;
-        }
-    } else if (w.mode == 1) {
-        const bool torus = w.border == 0;
-        const float ext[3] = {(float)W, (float)H, (float)D};
-        const float half[3] = {(float)(W / 2), (float)(H / 2), (float)(D / 2)};
-        const int dd = w.dd;
-        for (int dz = -dd; dz <= dd; dz++)
-            for (int dy = -dd; dy <= dd; dy++)
-                for (int dx = -dd; dx <= dd; dx++) {
-                    int sx = x + dx, sy = y + dy, sz = z + dz;
-                    if (!halo_src<W, H, D>(sx, sy, sz, tkind, faces)) continue;
-                    const long long s = sx + W * (sy + H * sz);
-#pragma unroll
-                    for (int c = 0; c < LHD_MAX_C; c++) {
-                        if (c >= nrt) break;
-                        const float *mu = MU + (long long)c * 3 * S::N;
-                        float m[3];
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            float dv = pos[d] - mu[(long long)d * S::N + s];
-                            if (torus) {
-                                if (dv > half[d]) dv = dv - ext[d];
-                                if (dv < -half[d]) dv = dv + ext[d];
-                            }
-                            dv = dv < 0.0f ? -dv : dv;
-                            dv = w.si - dv;
-                            dv = fmax_ref(dv, 0.0f);
-                            m[d] = fmin_ref(dv, w.clip);
-                        }
-                        float wt = ARR[(long long)c * S::N + s] * m[0];
-                        wt = wt * m[1];
-                        wt = wt * m[2];
-                        wt = wt * w.sn;
-                        tt[c] = tt[c] + wt;
-                    }
-                }
-    } else {
-        const float ext[3] = {(float)W, (float)H, (float)D};
-        const bool wrap = (w.quirks & LHQ_TRANSPORT_TORUS) || w.border == 0;
-#pragma unroll
-        for (int c = 0; c < LHD_MAX_C; c++) {
-            if (c >= C) break;
-            const float *dl = MU + (long long)c * 3 * S::N;
-            int I0[3];
-            float F0[3], F1[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                float p = (pos[d] - 0.5f) - dl[(long long)d * S::N + i];
-                if (wrap) {
-                    p = p + ext[d];
-                    p = p - floorf(p / ext[d]) * ext[d];
-                }
-                const float fl = floorf(p);
-                I0[d] = (int)fl;
-                F1[d] = p - fl;
-                F0[d] = 1.0f - F1[d];
-            }
-            const float *Ac = ARR + (long long)c * S::N;
-            float acc = 0.0f;
-#pragma unroll
-            for (int corner = 0; corner < 8; corner++) {
-                const int bx = corner & 1, by = (corner >> 1) & 1, bz = (corner >> 2) & 1;
-                int cx = I0[0] + bx, cy = I0[1] + by, cz = I0[2] + bz;
-                float v = 0.0f;
-                if (halo_src<W, H, D>(cx, cy, cz, tkind, faces)) v = Ac[cx + W * (cy + H * cz)];
-                float t = v * (bx ? F1[0] : F0[0]);
-                t = t * (by ? F1[1] : F0[1]);
-                t = t * (bz ? F1[2] : F0[2]);
-                acc = corner == 0 ? t : acc + t;
-            }
-            tt[c] = acc;
-        }
-    }
-}
-
-// End-of-step TRM for one cell (lenia.cpp:930-1035).
-template <int W, int H, int D>
-__device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, long long i, const float (&tt)[LHD_MAX_C]) {
-    using S = Shape<W, H, D>;
-    const int C = w.C;
-    const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
-    const float *Acur = A_cur(a, w);
-    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
-    const float *Hg = a.pool + w.Hg;
-    float ttsum = 0.0f;
-#pragma unroll
-    for (int c = 0; c < LHD_MAX_C; c++) {
-        if (c >= C) break;
-        const long long g = (long long)c * S::N + i;
-        float t = tt[c];
-        if (w.growth_type == 2) {
-            const bool soft = (w.soft_mask >> c) & 1;
-            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) a.pool[w.Bk + g] = soft_growth(Acur[g], Hg[g], w.dt);
-            else if (soft) t = soft_growth(t, Hg[g], w.dt);
-            else t = clamp01 ? clamp_ref(t + Hg[g] * w.dt, 0.0f, 1.0f) : t + Hg[g] * w.dt;
-        } else if (clamp01) {
-            t = clamp_ref(t, 0.0f, 1.0f);
-        }
-        ttsum = ttsum + t;
-        GA[g] = w.gtam * GA[g] + t * w.gtas;
-        An[g] = t;
-    }
-    const float sold = (w.quirks & LHQ_SUM_RESET) ? 0.0f : Sg[i];
-    Sg[i] = w.stam * sold + ttsum * w.stas;
-}
-
-// RT with dd = 1 from LDS: each staged source carries its nine per-axis overlap weights
-// (toward the destinations at offsets -1, 0, +1 along x, y and z), computed once with
-// exactly the expression the gather would evaluate for that (destination, source)
-// pair, so each destination's 27 products and their summation order are unchanged.
-// A slot's halo source is resolved once and all its global loads (mu, source mass for
-// every channel) are issued before any weight is formed.
-template <int W, int H, int D>
-__global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
-    using S = Shape<W, H, D>;
-    using T = TileCfg<W, H, D>;
-    constexpr int SPT = (T::NS + T::THREADS - 1) / T::THREADS;  // staged slots per thread
-    __shared__ float s_arr[T::NS];
-    __shared__ float s_w[9 * T::NS];
-    const int wi = blockIdx.x / T::TILES, tile = blockIdx.x % T::TILES;
-    const int ty0 = (tile % (H / T::TY)) * T::TY, tz0 = (tile / (H / T::TY)) * T::TZ;
-    const DevWorld &w = a.worlds[a.ids[wi]];
-    const int tid = threadIdx.x, C = w.C;
-    float tt[T::CPT][LHD_MAX_C];
-#pragma unroll
-    for (int j = 0; j < T::CPT; j++)
-#pragma unroll
-        for (int c = 0; c < LHD_MAX_C; c++) tt[j][c] = 0.0f;
-    if (w.mode == 1 && w.dd == 1) {
-        const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A_cur(a, w) : a.pool + w.Bk;
-        const float *MU = a.pool + w.MU;
-        const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
-        const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
-        const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
-        // rt_border_torus wraps the displacement by +-extent past half the extent;
-        // rt_border_other does not (an infinite threshold never triggers)
-        const bool torus = w.border == 0;
-        const float ext[3] = {(float)W, (float)H, (float)D};
-        const float half[3] = {torus ? (float)(W / 2) : __builtin_inff(), torus ? (float)(H / 2) : __builtin_inff(),
-                               torus ? (float)(D / 2) : __builtin_inff()};
-        const float si = w.si, clip = w.clip, sn = w.sn;
-        // gather every staged slot's inputs for all channels up front
-        float mu_r[SPT][LHD_MAX_C][3], arr_r[SPT][LHD_MAX_C];
-        int gpos[SPT][3];
-#pragma unroll
-        for (int j = 0; j < SPT; j++) {
-            const int q = tid + j * T::THREADS;
-            const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
-            gpos[j][0] = sx - 1;
-            gpos[j][1] = ty0 + sy - 1;
-            gpos[j][2] = tz0 + sz - 1;
-            int x = gpos[j][0], y = gpos[j][1], z = gpos[j][2];
-            const bool ok = q < T::NS && halo_src<W, H, D>(x, y, z, tkind, faces);
-            const int s = x + W * (y + H * z);
-#pragma unroll
-            for (int c = 0; c < LHD_MAX_C; c++) {
-                arr_r[j][c] = (ok && c < nrt) ? ARR[c * S::N + s] : 0.0f;
-#pragma unroll
-                for (int d = 0; d < 3; d++) mu_r[j][c][d] = (ok && c < nrt) ? MU[(c * 3 + d) * S::N + s] : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < LHD_MAX_C; c++) {
-            if (c >= nrt) break;
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < SPT; j++) {
-                const int q = tid + j * T::THREADS;
-                if (q >= T::NS) break;
-                s_arr[q] = arr_r[j][c];
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const float mu = mu_r[j][c][d];
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const float pos = (float)(gpos[j][d] - (k - 1)) + 0.5f;
-                        float dv = pos - mu;
-                        // the two one-sided wraps of rt_border_torus (reintegration.cpp:733-755):
-                        // after dv -= ext for dv > half, dv > -half, so at most one fires and
-                        // dv - (+-ext) equals the reference's dv - ext / dv + ext exactly
-                        const float wr = dv - __builtin_copysignf(ext[d], dv);
-                        dv = __builtin_fabsf(dv) > half[d] ? wr : dv;
-                        // abs_ps clears the sign bit; si - |dv| is never -0 or NaN here, so
-                        // v_max / v_min give the same bits as _mm_max_ps / _mm_min_ps
-                        dv = si - __builtin_fabsf(dv);
-                        s_w[(d * 3 + k) * T::NS + q] = __builtin_fminf(__builtin_fmaxf(dv, 0.0f), clip);
-                    }
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < T::CPT; j++) {
-                const int cell = tid + j * T::THREADS;
-                if (cell >= T::NT) break;
-                const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
-                float acc = tt[j][c];
-#pragma unroll
-                for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-                    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-                        for (int dx = -1; dx <= 1; dx++) {
-                            const int q = T::slot(tx + dx + 1, ty + dy + 1, tz + dz + 1);
-                            float wt = s_arr[q] * s_w[(dx + 1) * T::NS + q];
-                            wt = wt * s_w[(3 + dy + 1) * T::NS + q];
-                            wt = wt * s_w[(6 + dz + 1) * T::NS + q];
-                            wt = wt * sn;
-                            acc = acc + wt;
-                        }
-                tt[j][c] = acc;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < T::CPT; j++) {
-            const int cell = tid + j * T::THREADS;
-            if (cell >= T::NT) break;
-            const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
-            transport_cell<W, H, D>(a, w, tx, ty0 + ty, tz0 + tz, tt[j]);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < T::CPT; j++) {
-        const int cell = tid + j * T::THREADS;
-        if (cell >= T::NT) break;
-        const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
-        trm_cell<W, H, D>(a, w, tx + (long long)W * ((ty0 + ty) + (long long)H * (tz0 + tz)), tt[j]);
-    }
-}
-
-// ------------------------------------------------------------------------------------
 // Shape table.
 // ------------------------------------------------------------------------------------
 template <int W, int H, int D>
@@ -927,10 +415,12 @@ struct Launch {
 #define LH_OPS_ENTRY(w, h, d) Launch<w, h, d>::ops(),
 static const ShapeOps g_ops[] = {LH_SHAPES(LH_OPS_ENTRY)};
 
+const ShapeOps *lh_shape_ops_big(int W, int H, int D);
+
 const ShapeOps *lh_shape_ops(int W, int H, int D) {
     for (const ShapeOps &o : g_ops)
         if (o.W == W && o.H == H && o.D == D) return &o;
-    return nullptr;
+    return lh_shape_ops_big(W, H, D);
 }
 
 // ------------------------------------------------------------------------------------

@@ -70,6 +70,25 @@ def test_shapes_bitwise(shape, C, K, R):
         assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
 
 
+@pytest.mark.parametrize("shape,C,K,R,steps,over", [
+    ((64, 64, 64), 3, 9, 8, 3, dict(asymptotic=ASYM)),
+    ((64, 64, 64), 2, 4, 6, 2, dict(mode="adv", gradient="central", blur=True, sobel_border="none",
+                                    border="none", asymptotic=ASYM)),
+    # the 96^3 workload: C=4, K=16, R=12, central differences + blur, advection, zero border
+    ((96, 96, 96), 4, 16, 12, 2, dict(mode="adv", gradient="central", blur=True, sobel_border="none",
+                                      border="none", adv_vel=True, asymptotic=ASYM)),
+    ((128, 128, 128), 3, 9, 16, 1, dict(asymptotic=ASYM)),
+])
+def test_large_worlds_bitwise(shape, C, K, R, steps, over):
+    """Large worlds run the LDS-cooperative FFT kernels and 32-wide stencil tiles."""
+    w = WL.world(shape, C, K, R, seed=sum(shape) + steps, **over)
+    fields = ["A", "H", "S"] + (["MU"] if w["cfg"]["mode"] != "adv" else [])
+    r = run_pair(w, steps, fields=fields, keep_velocity=True)
+    for f in fields:
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
+
+
 @pytest.mark.parametrize("over", [{}, {"asymptotic": ASYM}])
 def test_config0_100_steps(over):
     """32^3, C=3, K=9, RT, Sobel, torus, growth table: 100 steps, bit-identical
