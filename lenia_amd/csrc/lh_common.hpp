@@ -18,7 +18,7 @@ struct DevWorld {
     int W, H, D, C, K;
     int mode, border, sobel_border, gradient, growth_type;
     int mu_vel, adv_vel, blur;
-    int keep_v, keep_usum, v_live, need_f;
+    int keep_v, keep_usum, v_live, need_f, need_alpha;
     int asym_mask, soft_mask;
     int osc_la, osc_flow, osc_div_flow, osc_vel;
     uint32_t quirks;
@@ -27,6 +27,7 @@ struct DevWorld {
     float osc_la_amt, osc_flow_amt, osc_div_flow_amt;
     // float offsets into the pool
     int64_t A, A2, GA, S, V, Bk, Hg, USUM, MU, Fv, Fsp, Gsp, KH, LUT, US, SUMS, BH;
+    int64_t AL;        // ALPHA = |A - GA| for the divergence oscillator [C][N]
     int64_t KST, KSP;  // staging (second pool, commit only): kernel fields and spectra
     int nkf;           // staged fields: K kernels (+1 blur kernel)
     DevKernel kern[LHD_MAX_K];

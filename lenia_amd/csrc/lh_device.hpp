@@ -56,6 +56,39 @@ __device__ __forceinline__ float *A_next(const PhaseArgs &a, const DevWorld &w) 
     return a.pool + (a.parity ? w.A : w.A2);
 }
 
+// exp for the analytic growth bell: Cody-Waite reduction x = n ln2 + r (fdlibm's split of
+// ln2 into a 33-bit head and a tail), the degree-13 Taylor polynomial of e^r in Horner
+// order with 1/k! rounded once, then scaling by 2^n.  Built without contraction, so the
+// CPU checker's identical sequence gives identical bits; relative error about 2e-16.
+__device__ __forceinline__ double canon_exp(double x) {
+    if (x < -700.0) return 0.0;
+    const double n = floor(x * 0x1.71547652b82fep+0 + 0.5);
+    const double r = (x - n * 0x1.62e42fee00000p-1) - n * 0x1.a39ef35793c76p-33;
+    double p = 0x1.6124613a86d09p-33;
+    p = p * r + 0x1.1eed8eff8d898p-29;
+    p = p * r + 0x1.ae64567f544e4p-26;
+    p = p * r + 0x1.27e4fb7789f5cp-22;
+    p = p * r + 0x1.71de3a556c734p-19;
+    p = p * r + 0x1.a01a01a01a01ap-16;
+    p = p * r + 0x1.a01a01a01a01ap-13;
+    p = p * r + 0x1.6c16c16c16c17p-10;
+    p = p * r + 0x1.1111111111111p-7;
+    p = p * r + 0x1.5555555555555p-5;
+    p = p * r + 0x1.5555555555555p-3;
+    p = p * r + 0x1.0000000000000p-1;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)n);
+}
+
+// Analytic growth bell(u, m, s) (render.h:674-677): quotient in fp32, square and exp in
+// double as the reference's pow() promotes, rounded once to fp32.
+__device__ __forceinline__ float bell_growth(float u, float m, float s) {
+    const float q = (u - m) / s;
+    const double d = (double)q;
+    return (float)canon_exp(-(d * d) / 2.0);
+}
+
 __device__ __forceinline__ float soft_growth(float g, float h, float dt) {
     // soft clip lenia.cpp:549-571 with fastexp reintegration.h:16-37
     const float a0 = 1.0f, a1 = 1.0f, a2 = 1.0f / 2, a3 = 1.0f / 2 / 3, a4 = 1.0f / 2 / 3 / 4;
@@ -203,6 +236,13 @@ __global__ __launch_bounds__(256) void k_local_tiled(PhaseArgs a, int) {
                 } else {
                     al = 0.0f;
                 }
+            }
+            if (w.need_alpha) {
+                // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
+                const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
+                const int gc = r / S::N;
+                const float ga = (r < LHD_MAX_C * S::N && gc < C) ? a.pool[w.GA + r] : 0.0f;
+                a.pool[w.AL + c * S::N + i] = __builtin_fabsf(A[c * S::N + i] - ga);
             }
             const float al1 = 1.0f - al;
             float F[3];
@@ -364,6 +404,50 @@ __device__ __forceinline__ void transport_cell(const PhaseArgs &a, const DevWorl
     }
 }
 
+// Optional oscillators (lenia.cpp:803-911), added to TT after the reintegration in the
+// reference's order: Laplacian of A, flow V.grad A (or F), divergence of V*ALPHA.
+template <int W, int H, int D>
+__device__ __forceinline__ void osc_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
+                                         float (&tt)[LHD_MAX_C]) {
+    using S = Shape<W, H, D>;
+    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+    const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
+    const float *G = A_cur(a, w);
+    const float *VF = a.pool + (w.osc_vel ? w.V : w.Fv);
+    const float *AL = a.pool + w.AL;
+    const int i = x + W * (y + H * z);
+    int nb[6];  // x+1, x-1, y+1, y-1, z+1, z-1 resolved through the transport halo (-1: zero)
+    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        int qx = x + off[k][0], qy = y + off[k][1], qz = z + off[k][2];
+        nb[k] = halo_src<W, H, D>(qx, qy, qz, tkind, faces) ? qx + W * (qy + H * qz) : -1;
+    }
+#pragma unroll
+    for (int c = 0; c < LHD_MAX_C; c++) {
+        if (c >= w.C) break;
+        const float *Gc = G + c * S::N;
+        auto g = [&](int k) { return nb[k] >= 0 ? Gc[nb[k]] : 0.0f; };
+        const float fx1 = g(0), fx0 = g(1), fy1 = g(2), fy0 = g(3), fz1 = g(4), fz0 = g(5);
+        if (w.osc_la) {
+            const float la = fx1 + fx0 + fy1 + fy0 + fz1 + fz0 - 6.0f * Gc[i];
+            tt[c] = tt[c] + la * w.osc_la_amt;
+        }
+        const float *V0 = VF + (c * 3 + 0) * S::N, *V1 = VF + (c * 3 + 1) * S::N, *V2 = VF + (c * 3 + 2) * S::N;
+        if (w.osc_flow) {
+            const float flow = V0[i] * (fx1 - fx0) + V1[i] * (fy1 - fy0) + V2[i] * (fz1 - fz0);
+            tt[c] = tt[c] + flow * w.osc_flow_amt;
+        }
+        if (w.osc_div_flow) {
+            const float *ALc = AL + c * S::N;
+            auto j = [&](const float *v, int k) { return nb[k] >= 0 ? v[nb[k]] * ALc[nb[k]] : 0.0f * 0.0f; };
+            const float jx = j(V0, 0) - j(V0, 1), jy = j(V1, 2) - j(V1, 3), jz = j(V2, 4) - j(V2, 5);
+            const float divj = jx + jy + jz;
+            tt[c] = tt[c] + divj * w.osc_div_flow_amt;
+        }
+    }
+}
+
 // End-of-step TRM for one cell (lenia.cpp:930-1035).
 template <int W, int H, int D>
 __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, long long i, const float (&tt)[LHD_MAX_C]) {
@@ -516,6 +600,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
         const int cell = tid + j * T::THREADS;
         if (cell >= T::NT) break;
         const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
+        if (w.osc_la || w.osc_flow || w.osc_div_flow) osc_cell<W, H, D>(a, w, tx0 + tx, ty0 + ty, tz0 + tz, tt[j]);
         trm_cell<W, H, D>(a, w, (tx0 + tx) + (long long)W * ((ty0 + ty) + (long long)H * (tz0 + tz)), tt[j]);
     }
 }

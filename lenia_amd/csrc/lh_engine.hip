@@ -244,7 +244,6 @@ lh_status validate(const lh_shape &s, const lh_params &p, const lh_kernel *k) {
     if (!(p.sigma > 0.0f) || !(p.dd >= 1.0f) || (int)p.dd > 4 || !(p.dd > p.sigma))
         return fail(LH_EINVAL, "need sigma > 0, 1 <= dd <= 4, dd > sigma");
     if (!(p.theta_a != 0.0f)) return fail(LH_EINVAL, "theta_a must be nonzero");
-    if (p.osc_la || p.osc_flow || p.osc_div_flow) return fail(LH_EUNSUPPORTED, "oscillators are not implemented");
     return LH_OK;
 }
 
@@ -257,8 +256,17 @@ DevWorld make_dev(const HostWorld &hw, const Engine &e) {
     d.growth_type = p.growth_type; d.mu_vel = p.mu_vel; d.adv_vel = p.adv_vel; d.blur = p.blur && p.gradient == 0;
     d.keep_v = e.opt_keep_v;
     d.keep_usum = e.opt_keep_usum;
-    d.v_live = p.mu_vel || (p.mode == LH_MODE_ADV && p.adv_vel) || e.opt_keep_v;
-    d.need_f = 0;
+    const bool osc_vf = p.osc_flow || p.osc_div_flow;
+    d.v_live = p.mu_vel || (p.mode == LH_MODE_ADV && p.adv_vel) || (osc_vf && p.osc_vel) || e.opt_keep_v;
+    d.need_f = osc_vf && !p.osc_vel;
+    d.need_alpha = p.osc_div_flow != 0;
+    d.osc_la = p.osc_la;
+    d.osc_flow = p.osc_flow;
+    d.osc_div_flow = p.osc_div_flow;
+    d.osc_vel = p.osc_vel;
+    d.osc_la_amt = p.osc_la_amt;
+    d.osc_flow_amt = p.osc_flow_amt;
+    d.osc_div_flow_amt = p.osc_div_flow_amt;
     for (int c = 0; c < LH_MAX_C; c++) {
         d.asym_mask |= (p.asymptotic[c] ? 1 : 0) << c;
         d.soft_mask |= (p.soft_clip[c] ? 1 : 0) << c;
@@ -522,6 +530,7 @@ lh_status lh_commit(void *p) {
         d.US = d.blur ? take(C * N) : d.Hg;
         d.SUMS = d.blur ? take(N) : d.S;
         d.BH = d.blur ? take(Noct) : d.KH;
+        d.AL = d.need_alpha ? take(C * N) : d.Hg;
         hw.dev = d;
         ShapeClass &c = e->classes[hw.cls];
         c.maxC = std::max(c.maxC, C);

@@ -270,8 +270,7 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
                     iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
                     g = slut[kk * 256 + iu];
                 } else {
-                    const float q = (u - kp.m) / kp.s;
-                    g = expf(-(q * q) / 2.0f);
+                    g = bell_growth(u, kp.m, kp.s);
                 }
 #pragma unroll
                 for (int c = 0; c < LHD_MAX_C; c++) {

@@ -209,8 +209,7 @@ __global__ __launch_bounds__(1024) void k_inv_growth_big(PhaseArgs a, int) {
                 iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
                 g = slut[iu];
             } else {
-                const float q = (u - kp.m) / kp.s;
-                g = expf(-(q * q) / 2.0f);
+                g = bell_growth(u, kp.m, kp.s);
             }
             if (w.keep_usum) US[co + i] += u;
             if (asym) Hg[co + i] += (g - A[co + i]) * kp.h;

@@ -102,6 +102,7 @@ enum {
     int  P##set_field(P##world *w, int field, const T *src);                              \
     int  P##get_field(P##world *w, int field, T *dst);                                    \
     void P##step(P##world *w, int nsteps);                                                \
+    void P##step_phases(P##world *w, int nphases);                                        \
     double P##mass(P##world *w);                                                          \
     void P##fft1(int n, T *re_im, int inverse);                                           \
     void P##rfft3(int W, int H, int D, const T *in, T *out);                              \
@@ -115,6 +116,7 @@ enum {
 
 LO_DECLARE(lo_f32_, float)
 LO_DECLARE(lo_f64_, double)
+double lo_canon_exp(double x);
 
 /* Edge mask of a marching-cubes case, generated from the cube's edge list. */
 int lo_edge_mask(int case_index);

@@ -478,6 +478,32 @@ static void fill_halo(world *w, real *a, int layers, int kind, int faces_only) {
             }
 }
 
+/* exp for the analytic growth bell: Cody-Waite reduction x = n ln2 + r (fdlibm's split
+ * of ln2 into a 33-bit head and a tail), the degree-13 Taylor polynomial of e^r in
+ * Horner order with 1/k! rounded once, then scaling by 2^n.  Written out operation by
+ * operation so that the HIP kernels evaluate the identical sequence; relative error
+ * about 2e-16 (pinned against numpy.exp). */
+static double canon_exp(double x) {
+    if (x < -700.0) return 0.0;
+    const double n = floor(x * 0x1.71547652b82fep+0 + 0.5);
+    const double r = (x - n * 0x1.62e42fee00000p-1) - n * 0x1.a39ef35793c76p-33;
+    double p = 0x1.6124613a86d09p-33;
+    p = p * r + 0x1.1eed8eff8d898p-29;
+    p = p * r + 0x1.ae64567f544e4p-26;
+    p = p * r + 0x1.27e4fb7789f5cp-22;
+    p = p * r + 0x1.71de3a556c734p-19;
+    p = p * r + 0x1.a01a01a01a01ap-16;
+    p = p * r + 0x1.a01a01a01a01ap-13;
+    p = p * r + 0x1.6c16c16c16c17p-10;
+    p = p * r + 0x1.1111111111111p-7;
+    p = p * r + 0x1.5555555555555p-5;
+    p = p * r + 0x1.5555555555555p-3;
+    p = p * r + 0x1.0000000000000p-1;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)n);
+}
+
 static real growth_value(const world *w, int k, real u) {
     const lo_kernel *kp = &w->kern[k];
     if (w->p.quirks & LO_Q_GROWTH_LUT) {
@@ -485,8 +511,10 @@ static real growth_value(const world *w, int k, real u) {
         iu = iu < 0 ? 0 : iu > 255 ? 255 : iu;
         return w->lut[(size_t)k * 256 + iu];
     }
+    /* bell(u, m, s) render.h:674-677: the square and exp in double, as pow() promotes */
     real q = (u - (real)kp->m) / (real)kp->s;
-    return (real)exp(-(q * q) / (real)2.0f);
+    double d = (double)q;
+    return (real)canon_exp(-(d * d) / 2.0);
 }
 
 /* Phase FFT + FFT_MT (lenia.cpp:435-594). */
@@ -907,6 +935,13 @@ static void phase_trm(world *w) {
                 w->SUM[pi] = stam * w->SUM[pi] + ttsum * stas;
             }
     forward_spectra(w);
+}
+
+/* The first nphases phases of one step (1 growth .. 8 TRM); 7 leaves TT readable. */
+void NAME(step_phases)(world *w, int nphases) {
+    void (*const ph[8])(world *) = {phase_growth, phase_blur, phase_tsb, phase_gradients_force,
+                                    phase_mu, phase_tb, phase_reintegration, phase_trm};
+    for (int k = 0; k < nphases && k < 8; k++) ph[k](w);
 }
 
 void NAME(step)(world *w, int nsteps) {

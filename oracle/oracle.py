@@ -79,6 +79,7 @@ def lib():
                 g.argtypes = [C.c_void_p, C.c_int, P]
                 g.restype = C.c_int
             getattr(L, pfx + "step").argtypes = [C.c_void_p, C.c_int]
+            getattr(L, pfx + "step_phases").argtypes = [C.c_void_p, C.c_int]
             m = getattr(L, pfx + "mass")
             m.argtypes = [C.c_void_p]
             m.restype = C.c_double
@@ -92,6 +93,8 @@ def lib():
             getattr(L, pfx + "mc_case").argtypes = [C.c_int] * 3 + [P, T, C.c_int, C.POINTER(C.c_uint8)]
         L.lo_edge_mask.argtypes = [C.c_int]
         L.lo_edge_mask.restype = C.c_int
+        L.lo_canon_exp.restype = C.c_double
+        L.lo_canon_exp.argtypes = [C.c_double]
         _lib = L
     return _lib
 
@@ -180,6 +183,10 @@ class World:
     def step(self, n=1):
         getattr(lib(), self.pfx + "step")(self._h, int(n))
 
+    def step_phases(self, n):
+        """Run the first n phases of one step (7: up to and including reintegration)."""
+        getattr(lib(), self.pfx + "step_phases")(self._h, int(n))
+
     def mass(self):
         return getattr(lib(), self.pfx + "mass")(self._h)
 
@@ -253,6 +260,12 @@ def mc_case(field: np.ndarray, iso=0.5, face_only=True, prec=32):
     getattr(lib(), pfx + "mc_case")(W, H, D, _ptr(a, ct), ct(iso), int(face_only),
                                     out.ctypes.data_as(C.POINTER(C.c_uint8)))
     return out.reshape(D, H, W)
+
+
+def canon_exp(x) -> np.ndarray:
+    """The checker's exp for the analytic growth bell, elementwise over x (float64)."""
+    f = lib().lo_canon_exp
+    return np.array([f(float(v)) for v in np.ravel(x)], dtype=np.float64).reshape(np.shape(x))
 
 
 def edge_mask(case_index: int) -> int:

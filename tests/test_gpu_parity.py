@@ -13,6 +13,7 @@ from tests.pairs import mismatch, run_pair
 pytestmark = pytest.mark.gpu
 
 ASYM = [1, 1, 1, 1]
+OSC = dict(osc_la=1, osc_flow=1, osc_div_flow=1, osc_la_amt=-0.02, osc_flow_amt=0.03, osc_div_flow_amt=-0.04)
 
 CASES = {
     "rt_sobel_torus": dict(seed=1),
@@ -31,6 +32,16 @@ CASES = {
     "soft_clip_transposition": dict(seed=20, growth="transposition", soft_clip=[1, 1, 0, 0]),
     "growth_none_dd2": dict(seed=21, growth="none", dd=2.0, sigma=0.8, asymptotic=ASYM),
     "s_norm_half": dict(seed=22, s_norm=0.5, sigma=0.55, asymptotic=ASYM),
+    # no reference quirks: analytic growth bell, true Sobel axes, full halos
+    "physics_rt_growth": dict(seed=27, quirks=0),
+    "physics_none_srt": dict(seed=28, quirks=0, mode="srt", growth="none"),
+    "physics_transposition_clip": dict(seed=29, quirks=0, growth="transposition", soft_clip=[1, 0, 1, 0]),
+    # oscillators (lenia.cpp:803-911): Laplacian, flow and divergence-of-flow terms
+    "osc_all_vel": dict(seed=23, asymptotic=ASYM, **OSC, osc_vel=1),
+    "osc_force_wall": dict(seed=24, border="wall", sobel_border="wall", asymptotic=ASYM, **OSC, osc_vel=0),
+    "osc_adv_zero": dict(seed=25, mode="adv", border="none", sobel_border="none", quirks=0x7EF,
+                         asymptotic=ASYM, **OSC, osc_vel=1),
+    "osc_physics_srt": dict(seed=26, mode="srt", quirks=0, growth="none", **OSC, osc_vel=0),
 }
 
 
@@ -85,6 +96,15 @@ def test_large_worlds_bitwise(shape, C, K, R, steps, over):
     fields = ["A", "H", "S"] + (["MU"] if w["cfg"]["mode"] != "adv" else [])
     r = run_pair(w, steps, fields=fields, keep_velocity=True)
     for f in fields:
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
+
+
+@pytest.mark.parametrize("shape,C,K,R", [((16, 8, 16), 3, 5, 3), ((64, 64, 64), 2, 4, 6)])
+def test_oscillators_shapes_bitwise(shape, C, K, R):
+    w = WL.world(shape, C, K, R, seed=sum(shape) + 7, asymptotic=ASYM, mu_vel=True, **OSC, osc_vel=1)
+    r = run_pair(w, 3, fields=["A", "GA", "V"])
+    for f in ("A", "GA", "V"):
         bad, err = mismatch(*r[f])
         assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
 

@@ -75,3 +75,33 @@ def test_blur_field(oracle):
     assert np.isclose(f[0, 0, 1] / f[0, 0, 0], g[1] / g[0], rtol=1e-6)
     assert np.isclose(f[2, 2, 2] / f[0, 0, 0], g[-1] / g[0], rtol=1e-6)
     assert f[3, 0, 0] == 0 and f[0, 0, 14] > 0                  # support |r| <= 2 on the torus
+
+
+def test_canon_exp_accuracy(oracle):
+    """The analytic-growth exp agrees with numpy.exp to ~1 ulp over the bell's range
+    (x = -q^2/2 <= 0), is exactly 1 at 0, and its fp32 rounding matches float32(exp(x))."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([-rng.uniform(0, 1, 4000) ** 3 * 700, -rng.uniform(0, 104, 4000),
+                        [0.0, -0.5, -1e-300, -np.log(2), -699.9, -700.5, -800.0]])
+    got = oracle.canon_exp(x)
+    ref = np.exp(x)
+    assert got[x == 0.0][0] == 1.0
+    assert (got[x < -700] == 0).all()
+    ok = x >= -700
+    rel = np.abs(got[ok] - ref[ok]) / ref[ok]
+    assert rel.max() <= 4e-16, rel.max()
+    f_ok = ok & (x > -104)
+    assert np.count_nonzero(got[f_ok].astype(np.float32) != ref[f_ok].astype(np.float32)) == 0
+
+
+def test_analytic_growth_is_reference_bell(oracle):
+    """Growth without the LUT is bell(u, m, s) (render.h:674-677) per kernel: a world with
+    one kernel whose H, read after a step, is h * bell(U) at U = 0 (empty world)."""
+    k = dict(c0=0, c1=0, radius=2, beta_size=1, b=[1.0], r_a=1.0, m=0.15, s=0.03, h=0.7)
+    w = oracle.World((8, 8, 8), 1, [k], WL.base_cfg(quirks=0, growth="none"), 32)
+    w.set(oracle.F_A, np.zeros(512, np.float32))
+    w.step(1)
+    H = w.get(oracle.F_H)
+    q = np.float32((np.float32(0.0) - np.float32(0.15)) / np.float32(0.03))
+    g = np.float32(math.exp(-(float(q) * float(q)) / 2.0))
+    assert np.all(H == np.float32(g * np.float32(0.7))), (H[0], g * 0.7)

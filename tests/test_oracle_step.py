@@ -204,3 +204,70 @@ def test_splitmix64_known_values():
     # published splitmix64 outputs for seed 0
     z = WL.splitmix64(0, 3)
     assert [int(v) for v in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def _shift(f, d, axis, kind):
+    """f at x+d along axis ([C][D][H][W] layout); outside the world: wrapped (torus),
+    the nearest edge cell (wall) or 0 (none) -- the transport halo (transposition.cpp)."""
+    n = f.shape[axis]
+    idx = np.arange(n) + d
+    if kind == "torus":
+        return np.take(f, idx % n, axis=axis)
+    out = np.take(f, np.clip(idx, 0, n - 1), axis=axis)
+    if kind == "none":
+        shape = [1] * f.ndim
+        shape[axis] = n
+        out = out * ((idx >= 0) & (idx < n)).reshape(shape)
+    return out
+
+
+@pytest.mark.parametrize("border,quirks,osc_vel", [("torus", PHYS, 1), ("none", PHYS, 0),
+                                                    ("none", 0x2 | 0x10, 1), ("wall", 0x2, 0)])
+def test_oscillators_closed_form(oracle, border, quirks, osc_vel):
+    """Oscillators (lenia.cpp:803-911): TT gains amt_la * Laplacian(A) + amt_flow * VF . dA +
+    amt_div * div(VF * ALPHA), ALPHA = |A - GA| (GA read at c*P+i under the index quirk)."""
+    W = H = D = 10
+    C = 2
+    st = WL.Stream(77)
+    A = st.u32(C * D * H * W).reshape(C, D, H, W).astype(np.float64)
+    GA = st.u32(C * D * H * W).reshape(C, D, H, W).astype(np.float64)
+    V0 = (st.u32(3 * C * D * H * W).astype(np.float64) - 0.5).reshape(C, 3, D, H, W)
+    amts = {k: float(np.float32(v)) for k, v in (("osc_la_amt", -0.03), ("osc_flow_amt", 0.05),
+                                                   ("osc_div_flow_amt", -0.07))}
+    tts, fields = [], {}
+    for on in (0, 1):
+        cfg = dict(growth="none", border=border, sobel_border=border, quirks=quirks, mu_vel=True,
+                   osc_la=on, osc_flow=on, osc_div_flow=on, osc_vel=osc_vel, **amts)
+        w = world(oracle, (W, H, D), C, [], **cfg)
+        w.set(oracle.F_A, A)
+        w.set(oracle.F_GA, GA)
+        w.set(oracle.F_V, V0)
+        w.step_phases(7)
+        tts.append(w.get(oracle.F_TT).reshape(C, D, H, W))
+        if on:
+            fields = dict(V=w.get(oracle.F_V).reshape(C, 3, D, H, W), F=w.get(oracle.F_F).reshape(C, 3, D, H, W))
+    kind = "torus" if quirks & 0x10 else border
+    VF = fields["V"] if osc_vel else fields["F"]
+    N, P = W * H * D, (W + 8) * (H + 8) * (D + 8)
+    GAr = np.zeros_like(GA)
+    flat = GA.reshape(-1)
+    for c in range(C):
+        idx = (c * P if quirks & 0x2 else c * N) + np.arange(N)
+        ok = idx < 4 * N
+        g = np.zeros(N)
+        g[ok] = np.concatenate([flat, np.zeros(4 * N - flat.size)])[idx[ok]]
+        GAr[c] = g.reshape(D, H, W)
+    ALPHA = np.abs(A - GAr)
+    axes = (3, 2, 1)  # x, y, z in [C][D][H][W]
+    lap = -6 * A
+    flow = np.zeros_like(A)
+    div = np.zeros_like(A)
+    for a, ax in enumerate(axes):
+        p1, m1 = _shift(A, 1, ax, kind), _shift(A, -1, ax, kind)
+        lap = lap + p1 + m1
+        flow = flow + VF[:, a] * (p1 - m1)
+        j = VF[:, a] * ALPHA
+        div = div + _shift(j, 1, ax, kind) - _shift(j, -1, ax, kind)
+    exp = amts["osc_la_amt"] * lap + amts["osc_flow_amt"] * flow + amts["osc_div_flow_amt"] * div
+    assert np.abs(exp).max() > 1e-3
+    assert np.allclose(tts[1] - tts[0], exp, atol=1e-12, rtol=0)
