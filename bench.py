@@ -5,6 +5,10 @@ periodic border, reference growth table), synthetic seeded worlds resident in HB
 next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
+                  [--no-mixed]
+
+A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
+per-world SRT or RT) and reports it under "mixed_batch".
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -30,6 +34,24 @@ from lenia_amd.engine import Engine  # noqa: E402
 from lenia_amd import roofline as RF  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+FP32_PEAK_TFS = 157.3  # MI355X FP32 vector, published
+PROFILE_SUMMARY = os.path.join(REPO, "profiles", "step_kernels_batch4096_32c3.md")
+
+
+def profiled_traffic(kernel, worlds):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), valid for the default
+    4096-world workload only; None when absent."""
+    if worlds != 4096 or not os.path.exists(PROFILE_SUMMARY):
+        return None
+    for ln in open(PROFILE_SUMMARY):
+        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cols) > 7 and cols[0] == kernel:
+            try:
+                return float(cols[7]) * 1e9
+            except ValueError:
+                return None
+    return None
 
 
 def build_engine(worlds, profile=False):
@@ -93,6 +115,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single-world leg")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-shape leg (24^3, 32^3, 36^3, 44x24x44, per-world SRT/RT)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -101,15 +125,14 @@ def main():
               "world_id mod G across GPUs; multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
-    seeds = list(range(100, 100 + min(args.worlds, 64)))
-    proto = [WL.plain32(seed=s) for s in seeds]
-    worlds = [proto[i % len(proto)] for i in range(args.worlds)]
+    worlds = [WL.plain32(seed=100 + i) for i in range(args.worlds)]
+    seeds = [100 + i for i in range(min(args.worlds, 16))]
 
     eng = build_engine(worlds)
     dt = timed(eng, args.steps, args.warmup)
     value = args.worlds * args.steps / dt
     ms_per_step = dt * 1e3 / args.steps
-    gpu_final = {seeds[i]: eng.get(i, "A") for i in range(min(len(seeds), 16))}
+    gpu_final = {seeds[i]: eng.get(i, "A") for i in range(len(seeds))}
     eng.close()
 
     single_rate, sdt = float("nan"), float("nan")
@@ -132,6 +155,21 @@ def main():
         "batched_over_single": value / single_rate,
     }
 
+    if not args.no_mixed:
+        mixed = WL.mixed(args.worlds, base_seed=1000)
+        meng = build_engine(mixed)
+        mdt = timed(meng, args.steps, args.warmup)
+        mass = meng.mass()
+        meng.close()
+        line["mixed_batch"] = {
+            "workload": f"{args.worlds} worlds, equal shares of 24^3, 32^3, 36^3, 44x24x44, C=3, K=9, "
+                        "per-world seeds and SRT/RT choice, one captured graph per step pair",
+            "value": args.worlds * args.steps / mdt, "unit": "world-steps/s",
+            "ms_per_step": mdt * 1e3 / args.steps,
+            "srt_worlds": sum(w["cfg"]["mode"] == "srt" for w in mixed),
+            "mass_finite": bool(np.isfinite(mass).all())}
+        del mixed
+
     if not args.no_profile:
         peng = build_engine(worlds, profile=True)
         peng.step(args.warmup)
@@ -148,8 +186,15 @@ def main():
         b = RF.kernel_bytes(dom, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds
         achieved = b / avg_s / 1e9
         line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                            "traffic": profiled_traffic(dom, args.worlds),
+                            "traffic_source": "profiles/step_kernels_batch4096_32c3.md (rocprofv3 --pmc "
+                                              "FETCH_SIZE / WRITE_SIZE, 2F+W after copy calibration)",
                             "algorithmic_bytes_per_launch": b, "avg_launch_us": avg_s * 1e6}
+        step_flops = RF.step_flops((32, 32, 32), 3, 9) * args.worlds
+        line["step_valu"] = {"canonical_flops_per_step": step_flops,
+                             "achieved_TFs": step_flops / (ms_per_step * 1e-3) / 1e12,
+                             "frac_of_published_fp32": step_flops / (ms_per_step * 1e-3) / 1e12 / FP32_PEAK_TFS}
         step_bytes = RF.step_bytes((32, 32, 32), 3, 9) * args.worlds
         line["step_roofline"] = {"algorithmic_bytes_per_step": step_bytes,
                                  "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
