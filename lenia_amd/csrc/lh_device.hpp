@@ -485,7 +485,8 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
 // pair, so each destination's 27 products and their summation order are unchanged.
 // A slot's halo source is resolved once and all its global loads (mu, source mass for
 // every channel) are issued before any weight is formed.
-template <int W, int H, int D>
+// OSC: compiled with the oscillator terms (used when a world of the batch enables one).
+template <int W, int H, int D, bool OSC>
 __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using T = TileCfg<W, H, D>;
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
         const int cell = tid + j * T::THREADS;
         if (cell >= T::NT) break;
         const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
-        if (w.osc_la || w.osc_flow || w.osc_div_flow) osc_cell<W, H, D>(a, w, tx0 + tx, ty0 + ty, tz0 + tz, tt[j]);
+        if (OSC && (w.osc_la || w.osc_flow || w.osc_div_flow)) osc_cell<W, H, D>(a, w, tx0 + tx, ty0 + ty, tz0 + tz, tt[j]);
         trm_cell<W, H, D>(a, w, (tx0 + tx) + (long long)W * ((ty0 + ty) + (long long)H * (tz0 + tz)), tt[j]);
     }
 }

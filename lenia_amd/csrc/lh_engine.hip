@@ -130,6 +130,7 @@ struct ShapeClass {
     int *d_ids = nullptr;
     int maxC = 0, maxNkf = 0;
     bool any_blur = false;
+    bool any_osc = false;
 };
 
 struct Prof {
@@ -218,7 +219,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "inv_blur", c.ops->inv_blur, a, 0);
         }
         run(e, "local", c.ops->local, a, 0);
-        run(e, "transport", c.ops->transport, a, 0);
+        run(e, "transport", c.ops->transport, a, c.any_osc ? 1 : 0);
         PhaseArgs an = args_for(e, c, parity ^ 1);
         run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));
     }
@@ -536,6 +537,7 @@ lh_status lh_commit(void *p) {
         c.maxC = std::max(c.maxC, C);
         c.maxNkf = std::max(c.maxNkf, d.nkf);
         c.any_blur |= d.blur != 0;
+        c.any_osc |= (d.osc_la || d.osc_flow || d.osc_div_flow);
     }
     e->pool_floats = (size_t)off;
     HIPCHK(hipMalloc(&e->pool, e->pool_floats * sizeof(float)));

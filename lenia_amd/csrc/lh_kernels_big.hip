@@ -272,9 +272,11 @@ struct LaunchBig {
         hipLaunchKernelGGL((k_local_tiled<W, H, D>), dim3(a.nworlds * TileCfg<W, H, D>::TILES), dim3(256), 0,
                            (hipStream_t)st, a, 0);
     }
-    static void transport(const PhaseArgs &a, int, void *st) {
-        hipLaunchKernelGGL((k_transport_tiled<W, H, D>), dim3(a.nworlds * TileCfg<W, H, D>::TILES), dim3(256), 0,
-                           (hipStream_t)st, a, 0);
+    // osc != 0: some world of the batch enables an oscillator
+    static void transport(const PhaseArgs &a, int osc, void *st) {
+        const dim3 grid(a.nworlds * TileCfg<W, H, D>::TILES);
+        if (osc) hipLaunchKernelGGL((k_transport_tiled<W, H, D, true>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else hipLaunchKernelGGL((k_transport_tiled<W, H, D, false>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
     }
     static constexpr ShapeOps ops() {
         return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z};
