@@ -99,6 +99,19 @@ enum {
     LH_OPT_PROFILE = 3        /* time every kernel launch with HIP events (default 0) */
 };
 
+/* Growth convolution of a world (lh_set_world_conv).  LH_CONV_FFT is the fp32 FFT
+ * path that matches the reference's FFTW convolution (lenia.cpp:435-538); the others
+ * evaluate the same convolution as dense DFT matrix products on the matrix cores with
+ * 16-bit operands and fp32 accumulation, split into hi + lo parts (three products) or
+ * unsplit (one product).  No reference counterpart: a precision/throughput option. */
+enum {
+    LH_CONV_FFT = 0,
+    LH_CONV_GEMM_BF16X3 = 1,
+    LH_CONV_GEMM_FP16X3 = 2,
+    LH_CONV_GEMM_BF16 = 3,
+    LH_CONV_GEMM_FP16 = 4
+};
+
 /* ---- engine lifetime --------------------------------------------------------- */
 lh_status lh_create(int device, void **engine);
 lh_status lh_destroy(void *engine);
@@ -107,6 +120,10 @@ lh_status lh_set_option(void *engine, int option, int value);
 /* Adds a world; returns its id.  Must precede lh_commit. */
 lh_status lh_add_world(void *engine, const lh_shape *shape, const lh_params *params,
                        const lh_kernel *kernels, int *world_id);
+/* Selects the growth convolution of a world added before lh_commit (default
+ * LH_CONV_FFT).  The matrix-core methods need a cube of side 32, 64 or 128 and no
+ * blur pass (LH_EUNSUPPORTED otherwise).  No reference counterpart. */
+lh_status lh_set_world_conv(void *engine, int world, int method);
 /* Allocates HBM for all worlds, builds kernel spectra and growth tables on the GPU. */
 lh_status lh_commit(void *engine);
 

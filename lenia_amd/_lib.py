@@ -27,7 +27,11 @@ Q_ALL = 0x7FF
 EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commit", "lh_set_field",
            "lh_get_field", "lh_field_device_ptr", "lh_state_updated", "lh_step", "lh_synchronize",
            "lh_stream", "lh_world_mass", "lh_mc_case", "lh_world_count", "lh_profile_count",
-           "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version")
+           "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version",
+           "lh_set_world_conv")
+
+# growth convolution methods (lh_set_world_conv)
+CONV = {"fft": 0, "bf16x3": 1, "fp16x3": 2, "bf16": 3, "fp16": 4}
 
 
 class Params(C.Structure):
@@ -79,6 +83,7 @@ def load():
         "lh_set_option": (i, [vp, i, i]),
         "lh_add_world": (i, [vp, C.POINTER(Shape), C.POINTER(Params), C.POINTER(Kernel), C.POINTER(i)]),
         "lh_commit": (i, [vp]),
+        "lh_set_world_conv": (i, [vp, i, i]),
         "lh_set_field": (i, [vp, i, i, C.POINTER(f), st]),
         "lh_get_field": (i, [vp, i, i, C.POINTER(f), st]),
         "lh_field_device_ptr": (i, [vp, i, i, C.POINTER(C.POINTER(f)), C.POINTER(st)]),

@@ -55,6 +55,7 @@ struct PhaseArgs {
     const int *ids;           // worlds of this shape class
     int nworlds;              // entries in ids
     const float2 *twW, *twH, *twD;  // twiddle tables (cos, sin) for the three lengths
+    const unsigned short *dft;      // DFT-as-GEMM matrices of the class (fragment order), or null
 };
 
 // Which real field fwd_xy transforms.
@@ -74,3 +75,11 @@ struct ShapeOps {
 };
 
 const ShapeOps *lh_shape_ops(int W, int H, int D);
+
+// Growth convolution as DFT-as-GEMM on matrix cores (cubic worlds of side N); arg of
+// fx/fy/fz: channels per world, of iz/iy: kernels per world.
+struct GemmOps {
+    int N, conv;
+    launch_fn fx, fy, fz, iz, iy, ix;
+};
+const GemmOps *lh_gemm_ops(int n, int conv);

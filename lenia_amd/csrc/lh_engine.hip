@@ -18,6 +18,7 @@
 void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st);
 void lh_launch_mc(const float *f, int W, int H, int D, float iso, int face_only, unsigned char *out, void *st);
 void lh_launch_copy(const void *src, void *dst, size_t bytes, void *st);
+void lh_build_dft_mats(int n, int conv, std::vector<uint16_t> &out);
 
 namespace {
 
@@ -121,6 +122,7 @@ struct HostWorld {
     std::vector<lh_kernel> k;
     DevWorld dev;
     int cls;
+    int conv = LH_CONV_FFT;
 };
 
 struct ShapeClass {
@@ -131,6 +133,10 @@ struct ShapeClass {
     int maxC = 0, maxNkf = 0;
     bool any_blur = false;
     bool any_osc = false;
+    int conv = LH_CONV_FFT;          // growth convolution of every world in the class
+    const GemmOps *gops = nullptr;   // matrix-core passes when conv != LH_CONV_FFT
+    unsigned short *d_dft = nullptr; // their DFT matrices
+    int maxK = 0;
 };
 
 struct Prof {
@@ -171,6 +177,7 @@ PhaseArgs args_for(Engine &e, ShapeClass &c, int parity, float *pool2 = nullptr)
     a.twW = e.d_tw + e.tw_off[c.W];
     a.twH = e.d_tw + e.tw_off[c.H];
     a.twD = e.d_tw + e.tw_off[c.D];
+    a.dft = c.d_dft;
     return a;
 }
 
@@ -203,6 +210,7 @@ void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg)
 
 void enqueue_spectra(Engine &e) {
     for (ShapeClass &c : e.classes) {
+        if (c.gops) continue;  // the matrix-core path transforms A at the start of each step
         PhaseArgs a = args_for(e, c, e.parity);
         run(e, "fwd_xy", c.ops->fwd_xy, a, SRC_A | (c.maxC << 8));
     }
@@ -211,6 +219,17 @@ void enqueue_spectra(Engine &e) {
 void enqueue_step(Engine &e, int parity) {
     for (ShapeClass &c : e.classes) {
         PhaseArgs a = args_for(e, c, parity);
+        if (c.gops) {
+            run(e, "dft_fx", c.gops->fx, a, c.maxC);
+            run(e, "dft_fy", c.gops->fy, a, c.maxC);
+            run(e, "dft_fz", c.gops->fz, a, c.maxC);
+            run(e, "dft_iz", c.gops->iz, a, c.maxK);
+            run(e, "dft_iy", c.gops->iy, a, c.maxK);
+            run(e, "dft_ix_growth", c.gops->ix, a, 0);
+            run(e, "local", c.ops->local, a, 0);
+            run(e, "transport", c.ops->transport, a, c.any_osc ? 1 : 0);
+            continue;
+        }
         run(e, "spec_z", c.ops->spec_z, a, 0);
         run(e, "inv_growth", c.ops->inv_growth, a, 0);
         if (c.any_blur) {
@@ -441,7 +460,10 @@ lh_status lh_destroy(void *p) {
     hipSetDevice(e->device);
     hipStreamSynchronize(e->stream);
     destroy_graphs(*e);
-    for (ShapeClass &c : e->classes) hipFree(c.d_ids);
+    for (ShapeClass &c : e->classes) {
+        hipFree(c.d_ids);
+        hipFree(c.d_dft);
+    }
     hipFree(e->pool);
     hipFree(e->d_worlds);
     hipFree(e->d_tw);
@@ -484,6 +506,23 @@ lh_status lh_add_world(void *p, const lh_shape *shape, const lh_params *params, 
     return LH_OK;
 }
 
+lh_status lh_set_world_conv(void *p, int world, int method) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    if (e->committed) return fail(LH_ESTATE, "the convolution must be chosen before lh_commit");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    if (method < LH_CONV_FFT || method > LH_CONV_GEMM_FP16) return fail(LH_EINVAL, "unknown convolution method");
+    HostWorld &hw = e->worlds[world];
+    if (method != LH_CONV_FFT) {
+        const lh_shape &s = hw.shape;
+        if (s.W != s.H || s.W != s.D || !lh_gemm_ops(s.W, method))
+            return fail(LH_EUNSUPPORTED, "matrix-core convolution needs a cube of side 32, 64 or 128");
+        if (hw.p.blur) return fail(LH_EUNSUPPORTED, "matrix-core convolution has no blur pass");
+    }
+    hw.conv = method;
+    return LH_OK;
+}
+
 lh_status lh_commit(void *p) {
     Engine *e = as_engine(p);
     if (!e) return fail(LH_EINVAL, "null engine");
@@ -495,11 +534,15 @@ lh_status lh_commit(void *p) {
         HostWorld &hw = e->worlds[i];
         int ci = -1;
         for (int j = 0; j < (int)e->classes.size(); j++)
-            if (e->classes[j].W == hw.shape.W && e->classes[j].H == hw.shape.H && e->classes[j].D == hw.shape.D) ci = j;
+            if (e->classes[j].W == hw.shape.W && e->classes[j].H == hw.shape.H && e->classes[j].D == hw.shape.D &&
+                e->classes[j].conv == hw.conv)
+                ci = j;
         if (ci < 0) {
             ShapeClass c;
             c.W = hw.shape.W; c.H = hw.shape.H; c.D = hw.shape.D;
             c.ops = lh_shape_ops(c.W, c.H, c.D);
+            c.conv = hw.conv;
+            if (hw.conv != LH_CONV_FFT) c.gops = lh_gemm_ops(c.W, hw.conv);
             e->classes.push_back(c);
             ci = (int)e->classes.size() - 1;
         }
@@ -536,6 +579,7 @@ lh_status lh_commit(void *p) {
         ShapeClass &c = e->classes[hw.cls];
         c.maxC = std::max(c.maxC, C);
         c.maxNkf = std::max(c.maxNkf, d.nkf);
+        c.maxK = std::max(c.maxK, K);
         c.any_blur |= d.blur != 0;
         c.any_osc |= (d.osc_la || d.osc_flow || d.osc_div_flow);
     }
@@ -560,6 +604,12 @@ lh_status lh_commit(void *p) {
     for (ShapeClass &c : e->classes) {
         HIPCHK(hipMalloc(&c.d_ids, c.ids.size() * sizeof(int)));
         HIPCHK(hipMemcpy(c.d_ids, c.ids.data(), c.ids.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (c.gops) {
+            std::vector<uint16_t> mats;
+            lh_build_dft_mats(c.W, c.conv, mats);
+            HIPCHK(hipMalloc(&c.d_dft, mats.size() * sizeof(uint16_t)));
+            HIPCHK(hipMemcpy(c.d_dft, mats.data(), mats.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        }
     }
     HIPCHK(hipMalloc(&e->d_mass, e->worlds.size() * sizeof(double)));
     lh_status s = build_kernel_spectra(*e);

@@ -41,13 +41,17 @@ class Engine:
     def __del__(self):
         self.close()
 
-    def add_world(self, shape, C_, kernels, cfg) -> int:
+    def add_world(self, shape, C_, kernels, cfg, conv="fft") -> int:
+        """conv: growth convolution, "fft" (fp32, reference-exact) or a matrix-core
+        DFT-as-GEMM variant "bf16x3", "fp16x3" (split hi+lo), "bf16", "fp16"."""
         W, H, D = shape
         s = L.Shape(W, H, D, C_, len(kernels))
         p = L.make_params(cfg)
         k = L.make_kernels(kernels)
         wid = C.c_int()
         L.check(self._L.lh_add_world(self._h, C.byref(s), C.byref(p), k, C.byref(wid)), "lh_add_world")
+        if conv != "fft":
+            L.check(self._L.lh_set_world_conv(self._h, wid.value, L.CONV[conv]), "lh_set_world_conv")
         self.worlds.append((tuple(shape), C_, len(kernels)))
         return wid.value
 

@@ -1,0 +1,163 @@
+"""The growth convolution as DFT-as-GEMM on the matrix cores (bf16/fp16 operands, fp32
+accumulation, optional hi+lo split), against the FP64 checker.
+
+Bar (floating point, one step from the same seeded state, analytic growth): U_SUM --
+the sum of the convolution outputs u that feed each channel's growth -- within 4e-5
+(bf16x3) / 1e-5 (fp16x3) max-abs of the FP64 checker, the same order as the fp32 FFT
+path (whose error is mostly the shared fp32 kernel spectrum); the unsplit variants are
+an order of magnitude worse, which is what the split buys.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from lenia_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+ANALYTIC = 0x7FF & ~0x40  # reference quirks, analytic growth bell instead of the 8-bit table
+
+
+def one_step(w, conv, fields=("USUM", "H", "A"), steps=1):
+    from lenia_amd.engine import Engine
+    eng = Engine(keep_usum=True)
+    wid = eng.add_world(w["shape"], w["C"], w["kernels"], w["cfg"], conv=conv)
+    eng.commit()
+    eng.set(wid, "A", w["A"])
+    eng.step(steps)
+    eng.synchronize()
+    out = {f: eng.get(wid, f).reshape(w["C"], -1).astype(np.float64) for f in fields}
+    eng.close()
+    return out
+
+
+def fp64(w, steps=1, fields=("USUM", "H", "A")):
+    from oracle import oracle as O
+    ow = O.World(w["shape"], w["C"], w["kernels"], w["cfg"], 64)
+    ow.set(O.F_A, w["A"])
+    ow.step(steps)
+    return {f: ow.get(getattr(O, "F_" + f)).reshape(w["C"], -1) for f in fields}
+
+
+def maxerr(a, b):
+    return float(np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("n,R", [(32, 8), (64, 12)])
+def test_split_variants_match_fp64(n, R):
+    w = WL.world((n, n, n), 3, 9, R, seed=n + 3, quirks=ANALYTIC)
+    ref = fp64(w)
+    errs = {conv: maxerr(one_step(w, conv)["USUM"], ref["USUM"]) for conv in ("fft", "bf16x3", "fp16x3")}
+    assert errs["bf16x3"] <= 4e-5, errs
+    assert errs["fp16x3"] <= 1e-5, errs
+    assert errs["fp16x3"] <= 2 * errs["fft"] + 1e-6, errs
+
+
+def test_unsplit_variants_are_coarser():
+    w = WL.world((32, 32, 32), 3, 9, 8, seed=35, quirks=ANALYTIC)
+    ref = fp64(w)
+    e = {conv: maxerr(one_step(w, conv)["USUM"], ref["USUM"]) for conv in ("bf16x3", "bf16", "fp16x3", "fp16")}
+    assert e["bf16"] <= 2e-2 and e["fp16"] <= 3e-3, e
+    assert e["bf16"] > 10 * e["bf16x3"] and e["fp16"] > 10 * e["fp16x3"], e
+
+
+def test_lut_growth_asymptotic_and_keep_usum():
+    """Reference growth table and asymptotic channels through the matrix-core path."""
+    w = WL.world((32, 32, 32), 3, 9, 8, seed=36, asymptotic=[1, 0, 1, 0])
+    ref = fp64(w)
+    got = one_step(w, "fp16x3")
+    assert maxerr(got["USUM"], ref["USUM"]) <= 1e-5
+    # growth-table bins can flip on 1e-6 differences; most cells agree to fp32 rounding
+    assert np.mean(np.abs(got["H"] - ref["H"]) <= 1e-5) > 0.999
+
+
+def test_physics_mode_mass_conserved():
+    from lenia_amd.engine import Engine
+    w = WL.world((64, 64, 64), 3, 9, 12, seed=40, quirks=0, growth="none")
+    eng = Engine()
+    wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"], conv="bf16x3")
+    eng.commit()
+    eng.set(wid, "A", w["A"])
+    m0 = eng.mass()[0]
+    eng.step(30)
+    m1 = eng.mass()[0]
+    eng.close()
+    assert abs(m1 - m0) / m0 <= 1e-5, (m0, m1)
+
+
+def test_mixed_engine_equals_single_worlds():
+    """Matrix-core and FFT worlds of different shapes in one engine step exactly as alone."""
+    from lenia_amd.engine import Engine
+    ws = [(WL.world((32, 32, 32), 3, 9, 8, seed=50, quirks=ANALYTIC), "bf16x3"),
+          (WL.world((64, 64, 64), 2, 4, 10, seed=51, quirks=ANALYTIC), "fp16x3"),
+          (WL.world((32, 32, 32), 3, 9, 8, seed=52, quirks=ANALYTIC), "fft"),
+          (WL.world((32, 32, 32), 2, 5, 8, seed=53, quirks=ANALYTIC, mode="srt"), "bf16x3")]
+    eng = Engine()
+    for w, conv in ws:
+        eng.add_world(w["shape"], w["C"], w["kernels"], w["cfg"], conv=conv)
+    eng.commit()
+    for i, (w, _) in enumerate(ws):
+        eng.set(i, "A", w["A"])
+    eng.step(4)
+    eng.synchronize()
+    batch = [eng.get(i, "A") for i in range(len(ws))]
+    eng.close()
+    for i, (w, conv) in enumerate(ws):
+        alone = one_step(w, conv, fields=("A",), steps=4)["A"].astype(np.float32).reshape(-1)
+        assert np.array_equal(batch[i].view(np.uint32), alone.view(np.uint32)), (i, conv)
+
+
+def test_rejects_unsupported():
+    from lenia_amd import _lib as L
+    from lenia_amd.engine import Engine
+    w = WL.world((44, 24, 44), 3, 9, 8, seed=1)
+    eng = Engine()
+    with pytest.raises(L.LeniaHipError, match="cube"):
+        eng.add_world(w["shape"], 3, w["kernels"], w["cfg"], conv="bf16x3")
+    w = WL.world((32, 32, 32), 3, 9, 8, seed=1, blur=True, gradient="central")
+    with pytest.raises(L.LeniaHipError, match="blur"):
+        eng.add_world(w["shape"], 3, w["kernels"], w["cfg"], conv="fp16x3")
+    eng.close()
+
+
+@pytest.mark.skipif(not os.environ.get("LENIA_PRECISION_128"),
+                    reason="128^3 x 50-step FP64 comparison (several minutes of CPU); "
+                           "set LENIA_PRECISION_128=1")
+def test_precision_statement_128():
+    """128^3, C=3, K=9, R=16, analytic growth bell, mass-conserving flow (physics mode,
+    growth drives the flow only -- with the reference's clamp and growth-into-A these
+    worlds saturate to A = 1 within 10 steps, which would make the comparison trivial):
+    per-channel max-abs error of A against the FP64 checker after 1, 10 and 50 steps for
+    the fp32 FFT and the matrix-core paths."""
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    w = WL.world((128, 128, 128), 3, 9, 16, seed=128, quirks=0, growth="none")
+    marks = (1, 10, 50)
+    ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 64)
+    ow.set(O.F_A, w["A"])
+    ref, done = {}, 0
+    for m in marks:
+        ow.step(m - done)
+        done = m
+        ref[m] = ow.get(O.F_A).reshape(3, -1)
+        moved = np.abs(ref[m] - w["A"].reshape(3, -1)).max(axis=1)
+        print(f"fp64 checker: {m} steps, max |A - A0| per channel {moved}, mass {ref[m].sum(axis=1)}",
+              flush=True)
+    rows = {}
+    for conv in ("fft", "bf16x3", "fp16x3", "bf16", "fp16"):
+        eng = Engine()
+        wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"], conv=conv)
+        eng.commit()
+        eng.set(wid, "A", w["A"])
+        done, errs = 0, []
+        for m in marks:
+            eng.step(m - done)
+            done = m
+            a = eng.get(wid, "A").reshape(3, -1).astype(np.float64)
+            errs.append(np.abs(a - ref[m]).max(axis=1))
+        eng.close()
+        rows[conv] = errs
+        print(conv, " | ".join(f"{m} steps: " + ", ".join(f"{v:.2e}" for v in e) for m, e in zip(marks, errs)),
+              flush=True)
+    assert rows["bf16x3"][0].max() <= 1e-3 and rows["fp16x3"][0].max() <= 1e-3
