@@ -5,10 +5,12 @@ periodic border, reference growth table), synthetic seeded worlds resident in HB
 next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
-                  [--no-mixed]
+                  [--no-mixed] [--no-precision]
 
 A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
-per-world SRT or RT) and reports it under "mixed_batch".
+per-world SRT or RT) and reports it under "mixed_batch"; a third steps one 128^3 world
+with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
+("precision_128", with the matrix-core roofline of its dominant pass).
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -35,6 +37,7 @@ from lenia_amd import roofline as RF  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 FP32_PEAK_TFS = 157.3  # MI355X FP32 vector, published
+BF16_PEAK_TFS = 2500.0  # MI355X dense bf16/fp16 matrix, published
 PROFILE_SUMMARY = os.path.join(REPO, "profiles", "step_kernels_batch4096_32c3.md")
 
 
@@ -106,6 +109,46 @@ def cpu_baseline(seeds, check_steps, budget_s=12.0):
     return base, {r[0]: r[1] for r in res}
 
 
+def precision_leg(steps, warmup):
+    """One 128^3 world (C=3, K=9, R=16, analytic growth bell, mass-conserving flow), its
+    growth convolution on the fp32 FFT path and as DFT-as-GEMM on the matrix cores."""
+    w = WL.world((128, 128, 128), 3, 9, 16, seed=128, quirks=0, growth="none")
+    paths = {}
+    for conv in ("fft", "bf16x3", "fp16x3"):
+        eng = Engine()
+        eng.add_world(w["shape"], 3, w["kernels"], w["cfg"], conv=conv)
+        eng.commit()
+        eng.set(0, "A", w["A"])
+        dt = timed(eng, steps, warmup)
+        eng.close()
+        paths[conv] = {"value": steps / dt, "unit": "world-steps/s", "ms_per_step": dt * 1e3 / steps}
+    peng = Engine(graph=False, profile=True)
+    peng.add_world(w["shape"], 3, w["kernels"], w["cfg"], conv="bf16x3")
+    peng.commit()
+    peng.set(0, "A", w["A"])
+    peng.step(warmup)
+    peng.synchronize()
+    peng.profile_reset()
+    peng.step(steps)
+    peng.synchronize()
+    prof = peng.profile()
+    peng.close()
+    gemm = {k: v for k, v in prof.items() if k.startswith("dft_")}
+    dom = max(gemm, key=lambda k: gemm[k][1])
+    n, ms = gemm[dom]
+    avg_s = ms / n * 1e-3
+    flops = RF.gemm_pass_flops(dom, 128) * RF.gemm_pass_fields(dom, 3, 9)
+    achieved = flops / avg_s / 1e12
+    return {"workload": "1 world 128^3, C=3, K=9, R=16, analytic growth, physics-mode flow",
+            "paths": paths,
+            "gemm_over_fft": paths["bf16x3"]["value"] / paths["fft"]["value"],
+            "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": BF16_PEAK_TFS,
+                         "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS, "traffic": None,
+                         "flops_per_launch": flops, "avg_launch_us": avg_s * 1e6},
+            "conv_flops_per_step": RF.gemm_step_flops(128, 3, 9),
+            "kernels": {k: {"launches": c, "avg_us": t * 1e3 / c} for k, (c, t) in prof.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +158,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single-world leg")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 128^3 leg (fp32 FFT vs DFT-as-GEMM bf16x3 / fp16x3 convolution)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-shape leg (24^3, 32^3, 36^3, 44x24x44, per-world SRT/RT)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -169,6 +214,9 @@ def main():
             "srt_worlds": sum(w["cfg"]["mode"] == "srt" for w in mixed),
             "mass_finite": bool(np.isfinite(mass).all())}
         del mixed
+
+    if not args.no_precision:
+        line["precision_128"] = precision_leg(args.steps, args.warmup)
 
     if not args.no_profile:
         peng = build_engine(worlds, profile=True)

@@ -9,6 +9,11 @@ Nh = (W/2+1) H D complex bins and real kernel octant Noct = (W/2+1)(H/2+1)(D/2+1
                             [+ 24 C N when the velocity is live (MU_VEL or ADV)]
   32^3, C=3, K=9: 963,300 B per world-step.
 * per kernel: every field the kernel must read or write, each exactly once.
+* DFT-as-GEMM convolution (cubic side n): matrix-core flops as issued, with P products
+  per multiply-accumulate (3 for the hi+lo split, 1 unsplit):
+    fx, ix (real n x n matrix on n^2 rows):          2 n^4 P           per channel / kernel
+    fy, fz, iz, iy (complex n x n on n (n/2+1) lines): 8 n^3 (n/2+1) P per channel / kernel
+  step: C (fx + fy + fz) + K (iz + iy + ix); 128^3, C=3, K=9, split: 97.8 GFLOP.
 """
 from __future__ import annotations
 
@@ -52,3 +57,22 @@ def kernel_bytes(kernel: str, shape, C, K, mode="rt", v_live=False, growth_rt=Tr
         # MU (3C) + source field (C) + GA r/w (2C) + A write (C) + S write
         return (3 * C + C + 2 * C + C) * N * F32 + N * F32
     raise KeyError(kernel)
+
+
+def gemm_pass_flops(kernel: str, n: int, split=True) -> float:
+    P = 3 if split else 1
+    if kernel in ("dft_fx", "dft_ix_growth"):
+        return 2.0 * n ** 4 * P
+    if kernel in ("dft_fy", "dft_fz", "dft_iz", "dft_iy"):
+        return 8.0 * n ** 3 * (n // 2 + 1) * P
+    raise KeyError(kernel)
+
+
+def gemm_step_flops(n: int, C: int, K: int, split=True) -> float:
+    f = lambda k: gemm_pass_flops(k, n, split)  # noqa: E731
+    return C * (f("dft_fx") + f("dft_fy") + f("dft_fz")) + K * (f("dft_iz") + f("dft_iy") + f("dft_ix_growth"))
+
+
+def gemm_pass_fields(kernel: str, C: int, K: int) -> int:
+    """Fields (channels or kernels) one launch of a pass processes."""
+    return C if kernel in ("dft_fx", "dft_fy", "dft_fz") else K
