@@ -146,6 +146,11 @@ void *lh_stream(void *engine);
 /* ---- reductions and read-backs ------------------------------------------------- */
 /* Total mass sum_c sum_i A, accumulated in FP64 per world, into out[n_worlds]. */
 lh_status lh_world_mass(void *engine, double *out, int n_worlds);
+/* Mass and circular first moments per world (FP64), out[7 * n_worlds]: sum A, then per
+ * axis x, y, z the sums of A cos(t) and A sin(t), t = 2 pi (i + 1/2) / extent, A summed
+ * over channels -- the centre of mass on the torus is atan2 of each pair.  No reference
+ * counterpart: labels for the learned parameter proposals. */
+lh_status lh_world_moments(void *engine, double *out, int n_worlds);
 /* Marching-cubes case index (mesh.cpp:173-185) of one channel of field LH_F_A,
  * LH_F_H or LH_F_USUM with the torus face pads of transpose_mesh_borders
  * (transposition.cpp:881-995): out[N] bytes. */

@@ -28,7 +28,9 @@ EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commi
            "lh_get_field", "lh_field_device_ptr", "lh_state_updated", "lh_step", "lh_synchronize",
            "lh_stream", "lh_world_mass", "lh_mc_case", "lh_world_count", "lh_profile_count",
            "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version",
-           "lh_set_world_conv")
+           "lh_set_world_conv", "lh_world_moments",
+           "lm_param_count", "lm_create", "lm_destroy", "lm_set_params", "lm_get_params", "lm_set_data",
+           "lm_train", "lm_gradients", "lm_forward", "lm_score_sampled", "lm_sample_features")
 
 # growth convolution methods (lh_set_world_conv)
 CONV = {"fft": 0, "bf16x3": 1, "fp16x3": 2, "bf16": 3, "fp16": 4}
@@ -92,6 +94,7 @@ def load():
         "lh_synchronize": (i, [vp]),
         "lh_stream": (vp, [vp]),
         "lh_world_mass": (i, [vp, C.POINTER(C.c_double), i]),
+        "lh_world_moments": (i, [vp, C.POINTER(C.c_double), i]),
         "lh_mc_case": (i, [vp, i, i, i, f, i, C.POINTER(C.c_uint8)]),
         "lh_world_count": (i, [vp]),
         "lh_profile_count": (i, [vp]),
@@ -100,6 +103,17 @@ def load():
         "lh_calibration_copy": (i, [vp, st]),
         "lh_shape_supported": (i, [i, i, i]),
         "lh_last_error": (C.c_char_p, []),
+        "lm_param_count": (st, []),
+        "lm_create": (i, [i, i, C.POINTER(vp)]),
+        "lm_destroy": (i, [vp]),
+        "lm_set_params": (i, [vp, C.POINTER(f), st]),
+        "lm_get_params": (i, [vp, C.POINTER(f), st]),
+        "lm_set_data": (i, [vp, C.POINTER(f), C.POINTER(f), i]),
+        "lm_train": (i, [vp, C.POINTER(i), i, i, f, C.POINTER(f)]),
+        "lm_gradients": (i, [vp, C.POINTER(f), C.POINTER(f), i, C.POINTER(f), C.POINTER(f)]),
+        "lm_forward": (i, [vp, C.POINTER(f), i, C.POINTER(f)]),
+        "lm_score_sampled": (i, [vp, C.c_uint64, C.c_longlong, C.POINTER(f)]),
+        "lm_sample_features": (i, [vp, C.c_uint64, C.c_longlong, i, C.POINTER(f)]),
         "lh_version": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():

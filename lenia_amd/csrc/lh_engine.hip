@@ -16,6 +16,7 @@
 #include "lh_common.hpp"
 
 void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st);
+void lh_launch_moments(const PhaseArgs &a, int nworlds, double *out, void *st);
 void lh_launch_mc(const float *f, int W, int H, int D, float iso, int face_only, unsigned char *out, void *st);
 void lh_launch_copy(const void *src, void *dst, size_t bytes, void *st);
 void lh_build_dft_mats(int n, int conv, std::vector<uint16_t> &out);
@@ -156,6 +157,7 @@ struct Engine {
     float2 *d_tw = nullptr;
     std::map<int, size_t> tw_off;
     double *d_mass = nullptr;
+    double *d_mom = nullptr;
     unsigned char *d_mc = nullptr;
     size_t mc_cap = 0;
     int parity = 0;
@@ -468,6 +470,7 @@ lh_status lh_destroy(void *p) {
     hipFree(e->d_worlds);
     hipFree(e->d_tw);
     hipFree(e->d_mass);
+    hipFree(e->d_mom);
     hipFree(e->d_mc);
     hipStreamDestroy(e->stream);
     delete e;
@@ -727,6 +730,21 @@ lh_status lh_world_mass(void *p, double *out, int n) {
     return LH_OK;
 }
 
+lh_status lh_world_moments(void *p, double *out, int n) {
+    Engine *e = as_engine(p);
+    if (!e || !out) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (n != (int)e->worlds.size()) return fail(LH_EINVAL, "n_worlds does not match");
+    HIPCHK(hipSetDevice(e->device));
+    if (!e->d_mom) HIPCHK(hipMalloc(&e->d_mom, 7 * e->worlds.size() * sizeof(double)));
+    PhaseArgs a = args_for(*e, e->classes[0], e->parity);
+    lh_launch_moments(a, n, e->d_mom, e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, e->d_mom, 7 * n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
 lh_status lh_mc_case(void *p, int world, int field, int channel, float iso, int face_only, uint8_t *out) {
     Engine *e = as_engine(p);
     if (!e || !out) return fail(LH_EINVAL, "null argument");
@@ -807,3 +825,6 @@ const char *lh_last_error(void) { return g_err.c_str(); }
 const char *lh_version(void) { return "lenia_hip 0.1 gfx950"; }
 
 }  // extern "C"
+
+// Error reporting for the other translation units of the library (lh_mlp.hip).
+lh_status lh_report_error(lh_status s, const std::string &msg) { return fail(s, msg); }

@@ -481,6 +481,51 @@ void lh_launch_copy(const void *src, void *dst, size_t bytes, void *st) {
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)st, (const float4 *)src, (float4 *)dst, n);
 }
 
+// Mass and first circular moments per world in FP64: out[7w..7w+6] = sum A, then for
+// x, y, z: sum A cos(theta), sum A sin(theta), theta = 2 pi (i + 1/2) / extent, A summed
+// over channels.  The angle tables are built once per block in LDS.
+__global__ __launch_bounds__(256) void k_moments(PhaseArgs a, double *out) {
+    const DevWorld &w = a.worlds[blockIdx.x];
+    const int W = w.W, H = w.H, D = w.D, C = w.C;
+    const long long N = (long long)W * H * D;
+    const float *A = a.pool + (a.parity ? w.A2 : w.A);
+    __shared__ double tab[2 * (128 + 128 + 128)];
+    double *cx = tab, *sx = tab + W, *cy = tab + 2 * W, *sy = cy + H, *cz = cy + 2 * H, *sz = cz + D;
+    for (int i = threadIdx.x; i < W + H + D; i += 256) {
+        const int ext = i < W ? W : (i < W + H ? H : D);
+        const int k = i < W ? i : (i < W + H ? i - W : i - W - H);
+        double sv, cv;
+        sincospi((2.0 * k + 1.0) / ext, &sv, &cv);
+        if (i < W) { cx[k] = cv; sx[k] = sv; }
+        else if (i < W + H) { cy[k] = cv; sy[k] = sv; }
+        else { cz[k] = cv; sz[k] = sv; }
+    }
+    __syncthreads();
+    double m[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long long i = threadIdx.x; i < N; i += 256) {
+        double v = 0.0;
+        for (int c = 0; c < C; c++) v += (double)A[c * N + i];
+        const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / ((long long)W * H));
+        m[0] += v;
+        m[1] += v * cx[x]; m[2] += v * sx[x];
+        m[3] += v * cy[y]; m[4] += v * sy[y];
+        m[5] += v * cz[z]; m[6] += v * sz[z];
+    }
+    __shared__ double red[7][256];
+    for (int k = 0; k < 7; k++) red[k][threadIdx.x] = m[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 7; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) out[7 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void lh_launch_moments(const PhaseArgs &a, int nworlds, double *out, void *st) {
+    hipLaunchKernelGGL(k_moments, dim3(nworlds), dim3(256), 0, (hipStream_t)st, a, out);
+}
+
 void lh_launch_mass(const PhaseArgs &a, int nworlds, double *out, void *st) {
     hipLaunchKernelGGL(k_mass, dim3(nworlds), dim3(256), 0, (hipStream_t)st, a, out);
 }

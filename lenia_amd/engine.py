@@ -102,6 +102,24 @@ class Engine:
                 "lh_world_mass")
         return out
 
+    def moments(self) -> np.ndarray:
+        """[n_worlds, 7] FP64: mass, then (sum A cos, sum A sin) for x, y, z."""
+        out = np.empty((len(self.worlds), 7), dtype=np.float64)
+        L.check(self._L.lh_world_moments(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), len(self.worlds)),
+                "lh_world_moments")
+        return out
+
+    def centre_of_mass(self) -> np.ndarray:
+        """[n_worlds, 3] torus centre of mass (x, y, z) in cells from moments()."""
+        m = self.moments()
+        ext = np.array([s[0] for s, _, _ in self.worlds]), np.array([s[1] for s, _, _ in self.worlds]), \
+            np.array([s[2] for s, _, _ in self.worlds])
+        out = np.empty((len(self.worlds), 3))
+        for a in range(3):
+            ang = np.arctan2(m[:, 2 + 2 * a], m[:, 1 + 2 * a])
+            out[:, a] = np.mod(ang / (2 * np.pi) * ext[a] - 0.5, ext[a])
+        return out
+
     def mc_case(self, wid, field="A", channel=0, iso=0.5, face_only=True) -> np.ndarray:
         (W, H, D), _, _ = self.worlds[wid]
         out = np.empty(W * H * D, dtype=np.uint8)

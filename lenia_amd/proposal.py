@@ -1,0 +1,139 @@
+"""Learned parameter proposals: sampler, features, rollouts and labels.
+
+A parameter set is one 32^3 world of the plain workload (C=3, K=9 kernels with
+c0 = k//3, c1 = k%3, R=8, r_a=1), drawn from the plain workload's ranges (1-3 rings with
+peaks U(0,1), m ~ U(0.1,0.4), s ~ U(0.02,0.1), h ~ U(0.2,1)).  Every draw is a pure
+function of (seed, set index, slot) through splitmix64, so a set chosen among 10^6 scored
+on the GPU is rebuilt exactly on the host for re-simulation.
+
+Features (81, in [0,1]): per kernel (c0/2, c1/2, (beta-1)/2, b0, b1, b2) for the 9 kernels,
+then per kernel ((m-0.1)/0.3, (s-0.02)/0.08, (h-0.2)/0.8).  Rings past beta carry b = 0.
+
+Rollouts run the batched engine in mass-conserving Flow Lenia (``FLOW``: no reference
+quirks, growth drives the flow only; with the reference's growth-into-A and clamp every
+world of these ranges fills the torus to A = 1 within 100 steps).  Mass being
+conserved, the labels of a rollout (``steps`` steps from a common initial box) are:
+survival (1 when the final mass concentration is at least ``ALIVE`` -- the creature
+stays as compact as the initial box, concentration 0.64, instead of dissolving), the
+final concentration itself, and the centre-of-mass drift on the torus in cells / W.
+Concentration is the smallest over the three axes of |sum A e^{i theta}| / sum A.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import workloads as WL
+
+GOLD = 0x9E3779B97F4A7C15
+MASK = 0xFFFFFFFFFFFFFFFF
+K, C, SHAPE, RADIUS = 9, 3, (32, 32, 32), 8
+N_FEATURES = 81
+SLOTS = 16           # draws per kernel: beta, b0..b2, m, s, h (7 used)
+ALIVE = 0.6          # survival: final mass concentration at least this
+FLOW = dict(quirks=0, growth="none")
+
+
+def draws(seed: int, sets: np.ndarray, slot: int) -> np.ndarray:
+    """U[0,1) multiples of 2^-24 for (seed, set, slot): splitmix64 of
+    seed*GOLD + set*256 + slot (mod 2^64)."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64((seed * GOLD) & MASK) + sets.astype(np.uint64) * np.uint64(256) + np.uint64(slot))
+        z = key + np.uint64(GOLD)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
+
+
+def sample(seed: int, sets) -> dict:
+    """Parameter arrays for the given set indices: beta [n,9] int, b [n,9,3], m, s, h [n,9]."""
+    sets = np.asarray(sets, dtype=np.int64)
+    n = sets.size
+    beta = np.empty((n, K), np.int64)
+    b = np.zeros((n, K, 3), np.float32)
+    m = np.empty((n, K), np.float32)
+    s = np.empty((n, K), np.float32)
+    h = np.empty((n, K), np.float32)
+    for k in range(K):
+        base = SLOTS * k
+        beta[:, k] = 1 + np.minimum((draws(seed, sets, base).astype(np.float64) * 3).astype(np.int64), 2)
+        for r in range(3):
+            u = draws(seed, sets, base + 1 + r)
+            b[:, k, r] = np.where(beta[:, k] > r, u, np.float32(0))
+        m[:, k] = np.float32(0.1) + draws(seed, sets, base + 4) * np.float32(0.3)
+        s[:, k] = np.float32(0.02) + draws(seed, sets, base + 5) * np.float32(0.08)
+        h[:, k] = np.float32(0.2) + draws(seed, sets, base + 6) * np.float32(0.8)
+    return dict(beta=beta, b=b, m=m, s=s, h=h)
+
+
+def features(p: dict) -> np.ndarray:
+    n = p["beta"].shape[0]
+    f = np.empty((n, N_FEATURES), np.float32)
+    for k in range(K):
+        f[:, 6 * k + 0] = (k // C) / 2.0
+        f[:, 6 * k + 1] = (k % C) / 2.0
+        f[:, 6 * k + 2] = (p["beta"][:, k] - 1) / 2.0
+        f[:, 6 * k + 3:6 * k + 6] = p["b"][:, k, :]
+        f[:, 54 + 3 * k + 0] = (p["m"][:, k] - np.float32(0.1)) / np.float32(0.3)
+        f[:, 54 + 3 * k + 1] = (p["s"][:, k] - np.float32(0.02)) / np.float32(0.08)
+        f[:, 54 + 3 * k + 2] = (p["h"][:, k] - np.float32(0.2)) / np.float32(0.8)
+    return f
+
+
+def kernels(p: dict, i: int) -> list:
+    """Engine kernel dicts of set i."""
+    out = []
+    for k in range(K):
+        beta = int(p["beta"][i, k])
+        out.append(dict(c0=k // C, c1=k % C, radius=RADIUS, beta_size=beta,
+                        b=[float(v) for v in p["b"][i, k, :beta]], r_a=1.0, m=float(p["m"][i, k]),
+                        s=float(p["s"][i, k]), h=float(p["h"][i, k])))
+    return out
+
+
+def initial_state(seed: int = 7) -> np.ndarray:
+    """The common initial field of every rollout: U(0,1) noise in the centred 16^3 box."""
+    return WL.box_init(WL.Stream(seed), SHAPE, C)
+
+
+def concentration(m: np.ndarray) -> np.ndarray:
+    """Smallest per-axis circular concentration from lh_world_moments rows [n, 7]."""
+    r = np.sqrt(m[:, 1::2] ** 2 + m[:, 2::2] ** 2) / np.maximum(m[:, :1], 1e-30)
+    return r.min(axis=1)
+
+
+def rollout(seed: int, sets, steps: int, batch: int = 4096, progress=None) -> dict:
+    """Simulates the sets on the batched engine; returns features and labels per set."""
+    from .engine import Engine
+    sets = np.asarray(sets, dtype=np.int64)
+    p = sample(seed, sets)
+    A0 = initial_state()
+    cfg = WL.base_cfg(**FLOW)
+    surv = np.empty(sets.size, np.float32)
+    conc = np.empty(sets.size, np.float32)
+    drift = np.empty(sets.size, np.float32)
+    for lo in range(0, sets.size, batch):
+        hi = min(sets.size, lo + batch)
+        eng = Engine()
+        for i in range(lo, hi):
+            eng.add_world(SHAPE, C, kernels(p, i), cfg)
+        eng.commit()
+        for j in range(hi - lo):
+            eng.set(j, "A", A0)
+        com0 = eng.centre_of_mass()
+        eng.step(steps)
+        r = concentration(eng.moments())
+        com1 = eng.centre_of_mass()
+        eng.close()
+        surv[lo:hi] = (r >= ALIVE).astype(np.float32)
+        conc[lo:hi] = r
+        d = np.abs(com1 - com0)
+        d = np.minimum(d, SHAPE[0] - d)
+        drift[lo:hi] = np.sqrt((d ** 2).sum(axis=1)) / SHAPE[0]
+        if progress:
+            progress(hi)
+    return dict(survival=surv, concentration=conc, drift=drift, features=features(p))
+
+
+def targets(r: dict) -> np.ndarray:
+    return np.stack([r["survival"], r["concentration"], r["drift"]], axis=1).astype(np.float32)

@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every symbol include/lenia_hip.h declares.
-No compute calls: this runs without a GPU."""
+"""The C-ABI library loads and exports every symbol include/lenia_hip.h and
+include/lenia_mlp.h declare.  No compute calls: this runs without a GPU."""
 import ctypes
 import os
 import re
@@ -9,15 +9,21 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared(header):
-    text = open(os.path.join(REPO, "include", header)).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(lh_[a-z_0-9]+)\s*\(", text)))
+def declared(*headers):
+    out = set()
+    for header in headers:
+        text = open(os.path.join(REPO, "include", header)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        out |= set(re.findall(r"\b(l[hm]_[a-z_0-9]+)\s*\(", text))
+    return sorted(out)
+
+
+HEADERS = ("lenia_hip.h", "lenia_mlp.h")
 
 
 def test_header_lists_match_binding():
     from lenia_amd import _lib
-    assert sorted(_lib.EXPORTS) == declared("lenia_hip.h")
+    assert sorted(_lib.EXPORTS) == declared(*HEADERS)
 
 
 def test_library_exports_every_declared_symbol():
@@ -25,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("liblenia_hip.so not built (run __graft_entry__.build())")
     lib = ctypes.CDLL(_lib.LIB_PATH)
-    missing = [s for s in declared("lenia_hip.h") if not hasattr(lib, s)]
+    missing = [s for s in declared(*HEADERS) if not hasattr(lib, s)]
     assert not missing, missing
 
 
