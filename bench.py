@@ -5,12 +5,14 @@ periodic border, reference growth table), synthetic seeded worlds resident in HB
 next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
-                  [--no-mixed] [--no-precision]
+                  [--no-mixed] [--no-precision] [--no-proposal]
 
 A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
 per-world SRT or RT) and reports it under "mixed_batch"; a third steps one 128^3 world
 with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
-("precision_128", with the matrix-core roofline of its dominant pass).
+("precision_128", with the matrix-core roofline of its dominant pass); a fourth times
+the proposal network ("proposal": scoring 10^6 sampled parameter sets in one call,
+training throughput).
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -149,6 +151,35 @@ def precision_leg(steps, warmup):
             "kernels": {k: {"launches": c, "avg_us": t * 1e3 / c} for k, (c, t) in prof.items()}}
 
 
+def proposal_leg():
+    """The proposal MLP: one call scoring 10^6 parameter sets sampled on the GPU, and
+    Adam steps of batch 256 on resident synthetic rows (throughput only)."""
+    from lenia_amd import mlp as M
+    from lenia_amd import proposal as P
+    net = M.MLP(max_batch=65536)
+    net.set_params(M.init_params(0))
+    net.score_sampled(1, 65536)
+    t0 = time.perf_counter()
+    net.score_sampled(2, 1_000_000)
+    t_score = time.perf_counter() - t0
+    x = P.features(P.sample(3, np.arange(16384)))
+    y = np.stack([(x[:, 54] > 0.5), x[:, 55], x[:, 58]], axis=1).astype(np.float32)
+    net.set_data(x, y)
+    net.train(1, 256, 1e-3, seed=0)
+    t0 = time.perf_counter()
+    net.train(4, 256, 1e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    net.close()
+    achieved = 1e6 * P.FLOPS_PER_SET / t_score / 1e12
+    return {"workload": "MLP 81-256-256-256-3 (GELU, bf16 on the matrix cores, fp32 master weights)",
+            "score_1e6_ms": t_score * 1e3, "score_sets_per_s": 1e6 / t_score,
+            "train_samples_per_s": 4 * 16384 / t_train,
+            "roofline": {"bound": "mfma", "kernel": "score_sampled (sampler + 4 GEMM layers)",
+                         "achieved": achieved, "peak": BF16_PEAK_TFS, "unit": "TFLOP/s",
+                         "frac": achieved / BF16_PEAK_TFS, "traffic": None,
+                         "flops_per_set": P.FLOPS_PER_SET}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +191,8 @@ def main():
     ap.add_argument("--no-single", action="store_true", help="skip the single-world leg")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 128^3 leg (fp32 FFT vs DFT-as-GEMM bf16x3 / fp16x3 convolution)")
+    ap.add_argument("--no-proposal", action="store_true",
+                    help="skip the proposal-network leg (scoring 10^6 sampled sets, training throughput)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-shape leg (24^3, 32^3, 36^3, 44x24x44, per-world SRT/RT)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -214,6 +247,9 @@ def main():
             "srt_worlds": sum(w["cfg"]["mode"] == "srt" for w in mixed),
             "mass_finite": bool(np.isfinite(mass).all())}
         del mixed
+
+    if not args.no_proposal:
+        line["proposal"] = proposal_leg()
 
     if not args.no_precision:
         line["precision_128"] = precision_leg(args.steps, args.warmup)

@@ -137,3 +137,74 @@ def rollout(seed: int, sets, steps: int, batch: int = 4096, progress=None) -> di
 
 def targets(r: dict) -> np.ndarray:
     return np.stack([r["survival"], r["concentration"], r["drift"]], axis=1).astype(np.float32)
+
+
+FLOPS_PER_SET = 2 * (81 * 256 + 256 * 256 * 2 + 256 * 3)  # forward, algorithmic
+
+
+def pipeline(rollouts: int, steps: int, score: int, top: int, epochs: int, batch: int = 256, lr: float = 1e-3,
+             seed: int = 100, log=print) -> dict:
+    """Rollouts -> training -> scoring `score` fresh sets in one call -> re-simulating the
+    `top` best and `top` uniform sets; returns the measurements."""
+    import time
+
+    from . import mlp as M
+    out = dict(rollouts=rollouts, steps=steps, scored=score, top=top, epochs=epochs, batch=batch, seed=seed)
+    t0 = time.perf_counter()
+    data = rollout(seed, np.arange(rollouts), steps,
+                   progress=lambda n: log(f"rollouts {n}/{rollouts} ({time.perf_counter() - t0:.0f} s)"))
+    t_roll = time.perf_counter() - t0
+    out["rollout_world_steps_per_s"] = rollouts * steps / t_roll
+    out["train_base_rate"] = float(data["survival"].mean())
+    x, y = data["features"], targets(data)
+    nval = rollouts // 10
+    net = M.MLP(max_batch=max(batch, 65536))
+    net.set_params(M.init_params(seed))
+    net.set_data(x[nval:], y[nval:])
+    t0 = time.perf_counter()
+    hist = []
+    for e in range(epochs):
+        hist += net.train(1, batch, lr, seed=seed + e)
+        if e % 10 == 9:
+            log(f"epoch {e + 1}: loss {hist[-1]:.4f}")
+    t_train = time.perf_counter() - t0
+    out["train_samples_per_s"] = epochs * (rollouts - nval) // batch * batch / t_train
+    out["train_loss"] = [hist[0], hist[-1]]
+    pv = net.forward(x[:nval])
+    out["val_accuracy"] = float(np.mean((pv[:, 0] > 0) == (y[:nval, 0] > 0.5)))
+    pred = pv[:, 0] > 0
+    out["val_precision"] = float(y[:nval, 0][pred].mean()) if pred.any() else 0.0
+    net.score_sampled(seed + 1, 65536)  # warm
+    t0 = time.perf_counter()
+    logits = net.score_sampled(seed + 1, score)
+    t_score = time.perf_counter() - t0
+    net.close()
+    out["score_sets_per_s"] = score / t_score
+    out["score_ms"] = t_score * 1e3
+    out["score_TFLOPs"] = score * FLOPS_PER_SET / t_score / 1e12
+    best = np.argsort(-logits)[:top]
+    hit = rollout(seed + 1, best, steps)
+    base = rollout(seed + 1, np.arange(top), steps)
+    out["hit_rate"] = float(hit["survival"].mean())
+    out["uniform_rate"] = float(base["survival"].mean())
+    out["gain"] = out["hit_rate"] / max(out["uniform_rate"], 1e-9)
+    return out
+
+
+def main():
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="learned parameter proposals: rollouts, training, scoring, re-simulation")
+    ap.add_argument("--rollouts", type=int, default=65536)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--score", type=int, default=1_000_000)
+    ap.add_argument("--top", type=int, default=4096)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=100)
+    a = ap.parse_args()
+    r = pipeline(a.rollouts, a.steps, a.score, a.top, a.epochs, seed=a.seed, log=lambda s: print(s, flush=True))
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
