@@ -509,15 +509,20 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
         const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
         const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
         const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
-        // rt_border_torus wraps the displacement by +-extent past half the extent;
-        // rt_border_other does not (an infinite threshold never triggers)
+        // rt_border_torus (reintegration.cpp:733-755) replaces the displacement dv by
+        // dv - ext when dv > ext/2 and by dv + ext when dv < -ext/2; rt_border_other never
+        // does.  With dd = 1 (ma = 1 - sigma < 1) every weight a destination reads has
+        // |dv| < 2 <= ext/2 when its source slot is an unwrapped neighbour and
+        // |dv| > ext - 2 >= ext/2 when the slot is a periodic image across the world's
+        // face (extents >= 4), so the test reduces to a per-slot, per-axis constant shift:
+        // +ext for an image below 0, -ext for one at or past the extent, else +0 (which
+        // leaves every dv unchanged, dv never being -0).
         const bool torus = w.border == 0;
         const float ext[3] = {(float)W, (float)H, (float)D};
-        const float half[3] = {torus ? (float)(W / 2) : __builtin_inff(), torus ? (float)(H / 2) : __builtin_inff(),
-                               torus ? (float)(D / 2) : __builtin_inff()};
+        const int extent[3] = {W, H, D};
         const float si = w.si, clip = w.clip, sn = w.sn;
         // gather every staged slot's inputs for all channels up front
-        float mu_r[SPT][LHD_MAX_C][3], arr_r[SPT][LHD_MAX_C];
+        float mu_r[SPT][LHD_MAX_C][3], arr_r[SPT][LHD_MAX_C], shift[SPT][3];
         int gpos[SPT][3];
 #pragma unroll
         for (int j = 0; j < SPT; j++) {
@@ -526,6 +531,9 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
             gpos[j][0] = tx0 + sx - 1;
             gpos[j][1] = ty0 + sy - 1;
             gpos[j][2] = tz0 + sz - 1;
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                shift[j][d] = !torus ? 0.0f : (gpos[j][d] < 0 ? ext[d] : (gpos[j][d] >= extent[d] ? -ext[d] : 0.0f));
             int x = gpos[j][0], y = gpos[j][1], z = gpos[j][2];
             const bool ok = q < T::NS && halo_src<W, H, D>(x, y, z, tkind, faces);
             const int s = x + W * (y + H * z);
@@ -552,11 +560,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
                     for (int k = 0; k < 3; k++) {
                         const float pos = (float)(gpos[j][d] - (k - 1)) + 0.5f;
                         float dv = pos - mu;
-                        // the two one-sided wraps of rt_border_torus (reintegration.cpp:733-755):
-                        // after dv -= ext for dv > half, dv > -half, so at most one fires and
-                        // dv - (+-ext) equals the reference's dv - ext / dv + ext exactly
-                        const float wr = dv - __builtin_copysignf(ext[d], dv);
-                        dv = __builtin_fabsf(dv) > half[d] ? wr : dv;
+                        dv = dv + shift[j][d];
                         // abs_ps clears the sign bit; si - |dv| is never -0 or NaN here, so
                         // v_max / v_min give the same bits as _mm_max_ps / _mm_min_ps
                         dv = si - __builtin_fabsf(dv);
