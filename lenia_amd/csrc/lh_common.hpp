@@ -33,6 +33,12 @@ struct DevWorld {
     DevKernel kern[LHD_MAX_K];
 };
 
+// Worlds whose transport runs as the z-sweep (RT, dd = 1, no oscillators); the tiled
+// transport kernel takes the rest.
+__host__ __device__ inline bool sweep_world(const DevWorld &w) {
+    return w.mode == 1 && w.dd == 1 && !(w.osc_la || w.osc_flow || w.osc_div_flow);
+}
+
 // quirk bits (same values as lenia_hip.h)
 #define LHQ_SOBEL_PERMUTE   (1u << 0)
 #define LHQ_PREALPHA_INDEX  (1u << 1)

@@ -479,6 +479,232 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
     Sg[i] = w.stam * sold + ttsum * w.stas;
 }
 
+// trm_cell split in two so the sweep can load a cell's inputs a step before it needs
+// them: trm_load reads them, trm_store applies the same arithmetic as trm_cell.
+struct TrmIn {
+    float ga[LHD_MAX_C], hg[LHD_MAX_C], ac[LHD_MAX_C], s;
+};
+
+template <int W, int H, int D>
+__device__ __forceinline__ void trm_load(const PhaseArgs &a, const DevWorld &w, long long i, TrmIn &in) {
+    using S = Shape<W, H, D>;
+    const float *Acur = A_cur(a, w), *GA = a.pool + w.GA, *Hg = a.pool + w.Hg;
+#pragma unroll
+    for (int c = 0; c < LHD_MAX_C; c++) {
+        const long long g = (long long)c * S::N + i;
+        const bool on = c < w.C;
+        in.ga[c] = on ? GA[g] : 0.0f;
+        in.hg[c] = (on && w.growth_type == 2) ? Hg[g] : 0.0f;
+        in.ac[c] = (on && w.growth_type == 2 && (w.quirks & LHQ_SOFTCLIP_TR)) ? Acur[g] : 0.0f;
+    }
+    in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : a.pool[w.S + i];
+}
+
+template <int W, int H, int D>
+__device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w, long long i,
+                                          const float (&tt)[LHD_MAX_C], const TrmIn &in) {
+    using S = Shape<W, H, D>;
+    const int C = w.C;
+    const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
+    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
+    float ttsum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < LHD_MAX_C; c++) {
+        if (c >= C) break;
+        const long long g = (long long)c * S::N + i;
+        float t = tt[c];
+        if (w.growth_type == 2) {
+            const bool soft = (w.soft_mask >> c) & 1;
+            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) a.pool[w.Bk + g] = soft_growth(in.ac[c], in.hg[c], w.dt);
+            else if (soft) t = soft_growth(t, in.hg[c], w.dt);
+            else t = clamp01 ? clamp_ref(t + in.hg[c] * w.dt, 0.0f, 1.0f) : t + in.hg[c] * w.dt;
+        } else if (clamp01) {
+            t = clamp_ref(t, 0.0f, 1.0f);
+        }
+        ttsum = ttsum + t;
+        GA[g] = w.gtam * in.ga[c] + t * w.gtas;
+        An[g] = t;
+    }
+    Sg[i] = w.stam * in.s + ttsum * w.stas;
+}
+
+// ------------------------------------------------------------------------------------
+// RT transport for dd = 1 as a sweep along z.  A workgroup owns a TX x TY column of
+// destinations through the whole depth.  Step z' stages source plane z' (one-cell x/y
+// halo) for every moved channel -- its mass and its nine overlap weights, each computed
+// once -- and adds the plane's nine terms per channel to the three destination planes
+// it touches: z'+1 (dz = -1), z' (dz = 0), z'-1 (dz = +1).  A destination therefore
+// receives its 27 terms in the reference's order (dz, dy, dx), and plane z'-1 is
+// complete after step z', when its TRM runs.  The product arr*wx*wy of a source is the
+// same for the three destinations and is formed once.
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D, int CM>
+struct SweepCfg {
+    static constexpr int TX = W <= 48 ? W : 32;
+    static constexpr int XT = W / TX;
+    // 8 rows when the plane's weights for 4 channels fit 64 KB of LDS, else 4
+    static constexpr int TY = (TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));
+    static constexpr int YT = H / TY;
+    static constexpr int THREADS = TX * TY;
+    static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
+    static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
+    static constexpr int BLOCKS = XT * YT;
+    static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // arr, wx0..2, wy0..2, wz0..2
+    static_assert(LHD_MAX_C * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
+};
+
+// One coordinate through the transport halo (halo_src, axis by axis): resolved index
+// and whether it was outside [0, n).
+__device__ __forceinline__ int halo_axis(int v, int n, int kind, int &out) {
+    out = (v < 0) | (v >= n);
+    if (kind == 0) return v < 0 ? v + n : (v >= n ? v - n : v);
+    return v < 0 ? 0 : (v >= n ? n - 1 : v);
+}
+
+// CM: channels moved, at most (3 or LHD_MAX_C).
+template <int W, int H, int D, int CM>
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_transport_sweep(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    using T = SweepCfg<W, H, D, CM>;
+    extern __shared__ float sw_dyn[];
+    const int wi = blockIdx.x / T::BLOCKS, b = blockIdx.x % T::BLOCKS;
+    const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    if (!sweep_world(w)) return;
+    const int tid = threadIdx.x, C = w.C;
+    const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A_cur(a, w) : a.pool + w.Bk;
+    const float *MU = a.pool + w.MU;
+    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
+    const int tkind = (w.quirks & LHQ_TRANSPORT_TORUS) ? 0 : w.border;
+    const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
+    // the torus distance wrap as a per-slot shift; see k_transport_tiled
+    const bool torus = w.border == 0;
+    const float si = w.si, clip = w.clip, sn = w.sn;
+    const int nitems = T::SP * nrt;
+    const int cx = tid % T::TX, cy = tid / T::TX;  // this thread's destination column
+    // z-independent part of every staged item: offsets of its (x, y) source and whether
+    // it is readable with z inside / outside the world (halo_src split per axis)
+    unsigned off[T::ITEMS];
+    bool ok_in[T::ITEMS], ok_out[T::ITEMS];
+    float posx[T::ITEMS], posy[T::ITEMS], shx[T::ITEMS], shy[T::ITEMS];
+#pragma unroll
+    for (int k = 0; k < T::ITEMS; k++) {
+        const int item = tid + k * T::THREADS;
+        const int c = item / T::SP, q = item % T::SP;
+        const int x = tx0 + q % T::SX - 1, y = ty0 + q / T::SX - 1;
+        int ox, oy;
+        const int xr = halo_axis(x, W, tkind, ox), yr = halo_axis(y, H, tkind, oy);
+        const int m = ox + oy;
+        const bool valid = item < nitems;
+        ok_in[k] = valid && (m == 0 || (tkind != 2 && !(faces && m >= 2)));
+        ok_out[k] = valid && tkind != 2 && !(faces && m + 1 >= 2);
+        off[k] = valid ? (unsigned)(c * 3 * S::N + xr + W * yr) : 0u;  // MU of channel c; mass at c*N
+        posx[k] = (float)x + 0.5f;
+        posy[k] = (float)y + 0.5f;
+        shx[k] = !torus ? 0.0f : (x < 0 ? (float)W : (x >= W ? -(float)W : 0.0f));
+        shy[k] = !torus ? 0.0f : (y < 0 ? (float)H : (y >= H ? -(float)H : 0.0f));
+    }
+    auto fetch = [&](int zp, float (&buf)[T::ITEMS][4]) {
+        int oz;
+        const int zr = halo_axis(zp, D, tkind, oz);
+        const unsigned zoff = (unsigned)(W * H * zr);
+#pragma unroll
+        for (int k = 0; k < T::ITEMS; k++) {
+            const bool ok = oz ? ok_out[k] : ok_in[k];
+            const unsigned o = off[k] + zoff;
+            const int c = (tid + k * T::THREADS) / T::SP;
+#pragma unroll
+            for (int d = 0; d < 3; d++) buf[k][d] = ok ? MU[o + d * S::N] : 0.0f;
+            buf[k][3] = ok ? ARR[o - 2 * c * S::N] : 0.0f;
+        }
+    };
+    float accP[CM], accC[CM], accN[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        accP[c] = 0.0f;
+        accC[c] = 0.0f;
+        accN[c] = 0.0f;
+    }
+    TrmIn pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
+    auto step = [&](int zp, const float (&cur)[T::ITEMS][4], float (&nxt)[T::ITEMS][4]) {
+        if (zp < D) fetch(zp + 1, nxt);
+        __syncthreads();  // the previous plane's weights are no longer read
+        const float shz = !torus ? 0.0f : (zp < 0 ? (float)D : (zp >= D ? -(float)D : 0.0f));
+        const float posz = (float)zp + 0.5f;
+#pragma unroll
+        for (int k = 0; k < T::ITEMS; k++) {
+            const int item = tid + k * T::THREADS;
+            if (item >= nitems) break;
+            const int c = item / T::SP, q = item % T::SP;
+            float *Wc = sw_dyn + c * T::LDS_FLOATS_PER_CH;
+            Wc[q] = cur[k][3];
+            const float p[3] = {posx[k], posy[k], posz}, sh[3] = {shx[k], shy[k], shz};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float mu = cur[k][d];
+#pragma unroll
+                for (int kk = 0; kk < 3; kk++) {
+                    // destination coordinate slot + 1 - kk, centre (float)(dest) + 0.5
+                    const float pos = kk == 1 ? p[d] : (kk == 0 ? p[d] + 1.0f : p[d] - 1.0f);
+                    float dv = pos - mu;
+                    dv = dv + sh[d];
+                    dv = si - __builtin_fabsf(dv);
+                    Wc[(1 + d * 3 + kk) * T::SP + q] = __builtin_fminf(__builtin_fmaxf(dv, 0.0f), clip);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CM; c++) {
+            if (c >= nrt) break;
+            // keep the scheduler from hoisting every channel's LDS reads at once (register
+            // pressure); an empty statement with a memory clobber, no instruction
+            asm volatile("" ::: "memory");
+            const float *Wc = sw_dyn + c * T::LDS_FLOATS_PER_CH;
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int q = (cx + dx + 1) + T::SX * (cy + dy + 1);
+                    float base = Wc[q] * Wc[(1 + dx + 1) * T::SP + q];
+                    base = base * Wc[(4 + dy + 1) * T::SP + q];
+                    float t = base * Wc[7 * T::SP + q];  // destination z'+1 sees this plane at dz = -1
+                    t = t * sn;
+                    accN[c] = accN[c] + t;
+                    t = base * Wc[8 * T::SP + q];        // destination z', dz = 0
+                    t = t * sn;
+                    accC[c] = accC[c] + t;
+                    t = base * Wc[9 * T::SP + q];        // destination z'-1, dz = +1
+                    t = t * sn;
+                    accP[c] = accP[c] + t;
+                }
+        }
+        const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
+        if (zp >= 1) {
+            float tt[LHD_MAX_C];
+#pragma unroll
+            for (int c = 0; c < LHD_MAX_C; c++) tt[c] = c < CM ? accP[c] : 0.0f;
+            trm_store<W, H, D>(a, w, col + (long long)W * H * (zp - 1), tt, pre);
+        }
+        if (zp >= 0 && zp < D) trm_load<W, H, D>(a, w, col + (long long)W * H * zp, pre);  // used next step
+#pragma unroll
+        for (int c = 0; c < CM; c++) {
+            accP[c] = accC[c];
+            accC[c] = accN[c];
+            accN[c] = 0.0f;
+        }
+    };
+    float cur[T::ITEMS][4], nxt[T::ITEMS][4];
+    fetch(-1, cur);
+    for (int zp = -1; zp <= D; zp++) {
+        step(zp, cur, nxt);
+#pragma unroll
+        for (int k = 0; k < T::ITEMS; k++)
+#pragma unroll
+            for (int d = 0; d < 4; d++) cur[k][d] = nxt[k][d];
+    }
+}
+
 // RT with dd = 1 from LDS: each staged source carries its nine per-axis overlap weights
 // (toward the destinations at offsets -1, 0, +1 along x, y and z), computed once with
 // exactly the expression the gather would evaluate for that (destination, source)
@@ -487,7 +713,7 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
 // every channel) are issued before any weight is formed.
 // OSC: compiled with the oscillator terms (used when a world of the batch enables one).
 template <int W, int H, int D, bool OSC>
-__global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
+__global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int skip_sweep) {
     using S = Shape<W, H, D>;
     using T = TileCfg<W, H, D>;
     constexpr int SPT = (T::NS + T::THREADS - 1) / T::THREADS;  // staged slots per thread
@@ -497,6 +723,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int) {
     const int tx0 = (tile % T::XT) * T::TX, trest = tile / T::XT;
     const int ty0 = (trest % (H / T::TY)) * T::TY, tz0 = (trest / (H / T::TY)) * T::TZ;
     const DevWorld &w = a.worlds[a.ids[wi]];
+    if (skip_sweep && sweep_world(w)) return;  // moved by k_transport_sweep
     const int tid = threadIdx.x, C = w.C;
     float tt[T::CPT][LHD_MAX_C];
 #pragma unroll

@@ -134,6 +134,7 @@ struct ShapeClass {
     int maxC = 0, maxNkf = 0;
     bool any_blur = false;
     bool any_osc = false;
+    bool any_sweep = false, any_other = false;  // transport by the z-sweep / by tiles
     int conv = LH_CONV_FFT;          // growth convolution of every world in the class
     const GemmOps *gops = nullptr;   // matrix-core passes when conv != LH_CONV_FFT
     unsigned short *d_dft = nullptr; // their DFT matrices
@@ -210,6 +211,10 @@ void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg)
     it->second.ms += ms;
 }
 
+int transport_arg(const ShapeClass &c) {
+    return (c.any_osc ? 1 : 0) | (c.any_sweep ? 2 : 0) | (c.any_other ? 4 : 0) | (c.maxC << 8);
+}
+
 void enqueue_spectra(Engine &e) {
     for (ShapeClass &c : e.classes) {
         if (c.gops) continue;  // the matrix-core path transforms A at the start of each step
@@ -229,7 +234,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "dft_iy", c.gops->iy, a, c.maxK);
             run(e, "dft_ix_growth", c.gops->ix, a, 0);
             run(e, "local", c.ops->local, a, 0);
-            run(e, "transport", c.ops->transport, a, c.any_osc ? 1 : 0);
+            run(e, "transport", c.ops->transport, a, transport_arg(c));
             continue;
         }
         run(e, "spec_z", c.ops->spec_z, a, 0);
@@ -240,7 +245,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "inv_blur", c.ops->inv_blur, a, 0);
         }
         run(e, "local", c.ops->local, a, 0);
-        run(e, "transport", c.ops->transport, a, c.any_osc ? 1 : 0);
+        run(e, "transport", c.ops->transport, a, transport_arg(c));
         PhaseArgs an = args_for(e, c, parity ^ 1);
         run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));
     }
@@ -585,6 +590,7 @@ lh_status lh_commit(void *p) {
         c.maxK = std::max(c.maxK, K);
         c.any_blur |= d.blur != 0;
         c.any_osc |= (d.osc_la || d.osc_flow || d.osc_div_flow);
+        (sweep_world(d) ? c.any_sweep : c.any_other) = true;
     }
     e->pool_floats = (size_t)off;
     HIPCHK(hipMalloc(&e->pool, e->pool_floats * sizeof(float)));

@@ -389,11 +389,28 @@ struct Launch {
         hipLaunchKernelGGL((k_local_tiled<W, H, D>), dim3(a.nworlds * TileCfg<W, H, D>::TILES), dim3(256), 0,
                            (hipStream_t)st, a, 0);
     }
-    // osc != 0: some world of the batch enables an oscillator
-    static void transport(const PhaseArgs &a, int osc, void *st) {
-        const dim3 grid(a.nworlds * TileCfg<W, H, D>::TILES);
-        if (osc) hipLaunchKernelGGL((k_transport_tiled<W, H, D, true>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
-        else hipLaunchKernelGGL((k_transport_tiled<W, H, D, false>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+    // arg bits: 1 some world enables an oscillator, 2 some world is moved by the
+    // z-sweep (RT, dd = 1, no oscillator), 4 some world is not; maxC << 8
+    static void transport(const PhaseArgs &a, int arg, void *st) {
+        if (arg & 2) {
+            const int maxc = arg >> 8;
+            if (maxc <= 3) {
+                using SC = SweepCfg<W, H, D, 3>;
+                hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS),
+                                   (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float), (hipStream_t)st, a, 0);
+            } else {
+                using SC = SweepCfg<W, H, D, LHD_MAX_C>;
+                hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS),
+                                   dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
+                                   (hipStream_t)st, a, 0);
+            }
+        }
+        if (arg & 4) {
+            const dim3 grid(a.nworlds * TileCfg<W, H, D>::TILES);
+            const int skip = (arg & 2) ? 1 : 0;
+            if (arg & 1) hipLaunchKernelGGL((k_transport_tiled<W, H, D, true>), grid, dim3(256), 0, (hipStream_t)st, a, skip);
+            else hipLaunchKernelGGL((k_transport_tiled<W, H, D, false>), grid, dim3(256), 0, (hipStream_t)st, a, skip);
+        }
     }
     static constexpr ShapeOps ops() {
         return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z};
