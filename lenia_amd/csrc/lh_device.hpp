@@ -481,43 +481,41 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
 
 // trm_cell split in two so the sweep can load a cell's inputs a step before it needs
 // them: trm_load reads them, trm_store applies the same arithmetic as trm_cell.
+// The growth-after-transport inputs (H, A) are read at use; GA and S are prefetched.
+template <int CM>
 struct TrmIn {
-    float ga[LHD_MAX_C], hg[LHD_MAX_C], ac[LHD_MAX_C], s;
+    float ga[CM], s;
 };
 
-template <int W, int H, int D>
-__device__ __forceinline__ void trm_load(const PhaseArgs &a, const DevWorld &w, long long i, TrmIn &in) {
+template <int W, int H, int D, int CM>
+__device__ __forceinline__ void trm_load(const PhaseArgs &a, const DevWorld &w, long long i, TrmIn<CM> &in) {
     using S = Shape<W, H, D>;
-    const float *Acur = A_cur(a, w), *GA = a.pool + w.GA, *Hg = a.pool + w.Hg;
+    const float *GA = a.pool + w.GA;
 #pragma unroll
-    for (int c = 0; c < LHD_MAX_C; c++) {
-        const long long g = (long long)c * S::N + i;
-        const bool on = c < w.C;
-        in.ga[c] = on ? GA[g] : 0.0f;
-        in.hg[c] = (on && w.growth_type == 2) ? Hg[g] : 0.0f;
-        in.ac[c] = (on && w.growth_type == 2 && (w.quirks & LHQ_SOFTCLIP_TR)) ? Acur[g] : 0.0f;
-    }
+    for (int c = 0; c < CM; c++) in.ga[c] = c < w.C ? GA[(long long)c * S::N + i] : 0.0f;
     in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : a.pool[w.S + i];
 }
 
-template <int W, int H, int D>
-__device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w, long long i,
-                                          const float (&tt)[LHD_MAX_C], const TrmIn &in) {
+// C <= CM for every world this is used on.
+template <int W, int H, int D, int CM>
+__device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w, long long i, const float (&tt)[CM],
+                                          const TrmIn<CM> &in) {
     using S = Shape<W, H, D>;
     const int C = w.C;
     const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
+    const float *Acur = A_cur(a, w), *Hg = a.pool + w.Hg;
     float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
     float ttsum = 0.0f;
 #pragma unroll
-    for (int c = 0; c < LHD_MAX_C; c++) {
+    for (int c = 0; c < CM; c++) {
         if (c >= C) break;
         const long long g = (long long)c * S::N + i;
         float t = tt[c];
         if (w.growth_type == 2) {
             const bool soft = (w.soft_mask >> c) & 1;
-            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) a.pool[w.Bk + g] = soft_growth(in.ac[c], in.hg[c], w.dt);
-            else if (soft) t = soft_growth(t, in.hg[c], w.dt);
-            else t = clamp01 ? clamp_ref(t + in.hg[c] * w.dt, 0.0f, 1.0f) : t + in.hg[c] * w.dt;
+            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) a.pool[w.Bk + g] = soft_growth(Acur[g], Hg[g], w.dt);
+            else if (soft) t = soft_growth(t, Hg[g], w.dt);
+            else t = clamp01 ? clamp_ref(t + Hg[g] * w.dt, 0.0f, 1.0f) : t + Hg[g] * w.dt;
         } else if (clamp01) {
             t = clamp_ref(t, 0.0f, 1.0f);
         }
@@ -625,7 +623,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_transport_
         accC[c] = 0.0f;
         accN[c] = 0.0f;
     }
-    TrmIn pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
+    TrmIn<CM> pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
     auto step = [&](int zp, const float (&cur)[T::ITEMS][4], float (&nxt)[T::ITEMS][4]) {
         if (zp < D) fetch(zp + 1, nxt);
         __syncthreads();  // the previous plane's weights are no longer read
@@ -680,13 +678,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_transport_
                 }
         }
         const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
-        if (zp >= 1) {
-            float tt[LHD_MAX_C];
-#pragma unroll
-            for (int c = 0; c < LHD_MAX_C; c++) tt[c] = c < CM ? accP[c] : 0.0f;
-            trm_store<W, H, D>(a, w, col + (long long)W * H * (zp - 1), tt, pre);
-        }
-        if (zp >= 0 && zp < D) trm_load<W, H, D>(a, w, col + (long long)W * H * zp, pre);  // used next step
+        if (zp >= 1) trm_store<W, H, D, CM>(a, w, col + (long long)W * H * (zp - 1), accP, pre);
+        if (zp >= 0 && zp < D) trm_load<W, H, D, CM>(a, w, col + (long long)W * H * zp, pre);  // used next step
 #pragma unroll
         for (int c = 0; c < CM; c++) {
             accP[c] = accC[c];

@@ -238,7 +238,7 @@ void enqueue_step(Engine &e, int parity) {
             continue;
         }
         run(e, "spec_z", c.ops->spec_z, a, 0);
-        run(e, "inv_growth", c.ops->inv_growth, a, 0);
+        run(e, "inv_growth", c.ops->inv_growth, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8));
         if (c.any_blur) {
             run(e, "fwd_xy_blur", c.ops->fwd_xy, a, SRC_HBLUR | ((c.maxC + 1) << 8));
             run(e, "spec_z_blur", c.ops->spec_z, a, 1);

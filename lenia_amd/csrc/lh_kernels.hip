@@ -186,31 +186,31 @@ struct InvCfg {
 };
 
 
-template <int W, int H, int D>
+// CM: channels per world at most (3 or 4); KU: U_SUM is written.
+template <int W, int H, int D, int CM, bool KU>
 __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using Cfg = InvCfg<W, H, D>;
     constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
     constexpr int CPT = (PL + T - 1) / T;  // cells of the plane owned by each thread
+    constexpr int CU = KU ? CM : 1;        // U_SUM accumulators
     __shared__ float2 sg[KB * Cfg::PLANE];
-    __shared__ float slut[KB * 256];
     const int wi = blockIdx.x / D, z = blockIdx.x % D;
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C, K = w.K;
     const bool lut = (w.quirks & LHQ_GROWTH_LUT) != 0;
+    // growth tables are read through the cache (1 KB per kernel, reused by every cell)
+    const float *glut = a.pool + w.LUT;
     const float *A = A_cur(a, w) + (long long)z * PL;
-    // per-cell accumulators live in registers: h[j][c], U_SUM us[j][c], and A for the
-    // asymptotic form, for the CPT cells tid + j*T of this plane
-    float h[CPT][LHD_MAX_C], us[CPT][LHD_MAX_C], av[CPT][LHD_MAX_C];
+    // per-cell accumulators live in registers: h[j][c] and U_SUM us[j][c] for the CPT
+    // cells tid + j*T of this plane; A is read through the cache where it is needed
+    float h[CPT][CM], us[CPT][CU];
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-        const int i = tid + j * T;
 #pragma unroll
-        for (int c = 0; c < LHD_MAX_C; c++) {
-            h[j][c] = 0.0f;
-            us[j][c] = 0.0f;
-            av[j][c] = (c < C && i < PL) ? A[(long long)c * S::N + i] : 0.0f;
-        }
+        for (int c = 0; c < CM; c++) h[j][c] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < CU; c++) us[j][c] = 0.0f;
     }
     const float2 *G = reinterpret_cast<const float2 *>(a.pool + w.Gsp);
     float *sgf = reinterpret_cast<float *>(sg);
@@ -221,8 +221,6 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
             const int kk = i / Cfg::PLANE, j = i % Cfg::PLANE;
             sg[i] = G[(long long)(k0 + kk) * S::Nh + (long long)z * Cfg::PLANE + j];
         }
-        if (lut)
-            for (int i = tid; i < nk * 256; i += T) slut[i] = a.pool[w.LUT + (long long)k0 * 256 + i];
         __syncthreads();
         for (int t = tid; t < nk * Wh; t += T) {
             const int kk = t / Wh, kx = t % Wh;
@@ -268,16 +266,16 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
                 if (lut) {
                     int iu = (int)(u * 255.0f);
                     iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
-                    g = slut[kk * 256 + iu];
+                    g = glut[(k0 + kk) * 256 + iu];
                 } else {
                     g = bell_growth(u, kp.m, kp.s);
                 }
+                const float hk = asym ? (g - A[(long long)c1 * S::N + i]) * kp.h : g * kp.h;
 #pragma unroll
-                for (int c = 0; c < LHD_MAX_C; c++) {
+                for (int c = 0; c < CM; c++) {
                     if (c != c1) continue;
-                    us[j][c] += u;
-                    if (asym) h[j][c] += (g - av[j][c]) * kp.h;
-                    else h[j][c] += g * kp.h;
+                    if (KU) us[j][c] += u;
+                    h[j][c] += hk;
                 }
             }
         }
@@ -289,14 +287,16 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
         const int i = tid + j * T;
         if (i >= PL) break;
 #pragma unroll
-        for (int c = 0; c < LHD_MAX_C; c++) {
+        for (int c = 0; c < CM; c++) {
             if (c >= C) break;
             const long long g = (long long)c * S::N + i;
             const float hv = h[j][c];
             Hg[g] = hv;
-            if (w.keep_usum) US[g] = us[j][c];
-            if (w.growth_type == 1)
-                B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(av[j][c], hv, w.dt) : clamp_ref(av[j][c] + hv * w.dt, 0.0f, 1.0f);
+            if (KU) US[g] = us[j][c];
+            if (w.growth_type == 1) {
+                const float av = A[g];
+                B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(av, hv, w.dt) : clamp_ref(av + hv * w.dt, 0.0f, 1.0f);
+            }
         }
     }
 }
@@ -379,8 +379,14 @@ struct Launch {
         constexpr int BPW = ((W / 2 + 1) * H + 127) / 128;
         hipLaunchKernelGGL((k_kspec_z<W, H, D>), dim3(a.nworlds * BPW), dim3(128), 0, (hipStream_t)st, a, 0);
     }
-    static void inv_growth(const PhaseArgs &a, int, void *st) {
-        hipLaunchKernelGGL((k_inv_growth<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
+    // arg: bit 0 U_SUM kept; maxC << 8
+    static void inv_growth(const PhaseArgs &a, int arg, void *st) {
+        const dim3 grid(a.nworlds * D);
+        const bool ku = arg & 1, c3 = (arg >> 8) <= 3;
+        if (c3 && !ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, false>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else if (c3) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, true>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else if (!ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, false>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, true>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
     }
     static void inv_blur(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_inv_blur<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
