@@ -126,172 +126,94 @@ struct TileCfg {
     __device__ static __forceinline__ int slot(int tx, int ty, int tz) { return tx + SX * (ty + SY * tz); }
 };
 
-// Sobel from a staged tile; taps in the reference order dz, dy, dx.  Tap weights are
-// +-1, +-2, +-4, so each product is exact and fma(v, c, g) rounds exactly like the
-// reference's separate multiply and add.
-template <int SX, int SY, bool PERM>
-__device__ __forceinline__ void sobel_lds(const float *sf, int tx, int ty, int tz, float (&g)[3]) {
-    constexpr int SW[3] = {1, 2, 1};
-    constexpr int SD[3] = {-1, 0, 1};
-    g[0] = 0.0f; g[1] = 0.0f; g[2] = 0.0f;
-#pragma unroll
-    for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const int gx = SD[dx + 1] * SW[dy + 1] * SW[dz + 1];
-                const int gy = SW[dx + 1] * SD[dy + 1] * SW[dz + 1];
-                const int gz = SW[dx + 1] * SW[dy + 1] * SD[dz + 1];
-                const int c0 = PERM ? gz : gx, c1 = PERM ? gx : gy, c2 = PERM ? gy : gz;
-                if (c0 == 0 && c1 == 0 && c2 == 0) continue;
-                const float v = sf[(tx + dx + 1) + SX * ((ty + dy + 1) + SY * (tz + dz + 1))];
-                if (c0 != 0) g[0] = __builtin_fmaf(v, (float)c0, g[0]);
-                if (c1 != 0) g[1] = __builtin_fmaf(v, (float)c1, g[1]);
-                if (c2 != 0) g[2] = __builtin_fmaf(v, (float)c2, g[2]);
-            }
-}
-
-template <int SX, int SY>
-__device__ __forceinline__ void cdiff_lds(const float *sf, int tx, int ty, int tz, float (&g)[3]) {
-    auto at = [&](int x, int y, int z) { return sf[(x + 1) + SX * ((y + 1) + SY * (z + 1))]; };
-    g[0] = (at(tx + 1, ty, tz) - at(tx - 1, ty, tz)) * 0.5f;
-    g[1] = (at(tx, ty + 1, tz) - at(tx, ty - 1, tz)) * 0.5f;
-    g[2] = (at(tx, ty, tz + 1) - at(tx, ty, tz - 1)) * 0.5f;
-}
-
-template <int SX, int SY>
-__device__ __forceinline__ void gradient_lds(const DevWorld &w, const float *sf, int tx, int ty, int tz, float (&g)[3]) {
-    if (w.gradient == 1) {
-        if (w.quirks & LHQ_SOBEL_PERMUTE) sobel_lds<SX, SY, true>(sf, tx, ty, tz, g);
-        else sobel_lds<SX, SY, false>(sf, tx, ty, tz, g);
-    } else {
-        cdiff_lds<SX, SY>(sf, tx, ty, tz, g);
-    }
-}
-
-template <int W, int H, int D>
-__global__ __launch_bounds__(256) void k_local_tiled(PhaseArgs a, int) {
+// Gradients -> alpha, F, mu, V for one cell (sobel_convolve_h lenia.cpp:647-672,
+// compute_f lenia.cpp:679-719, compute_*_mu reintegration.cpp:321-602); grad(f, g)
+// returns the gradient of staged field f (0: S or SUM_S, 1 + c: H_c or U_S_c).
+template <int W, int H, int D, class Grad>
+__device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z, Grad grad) {
     using S = Shape<W, H, D>;
-    using T = TileCfg<W, H, D>;
-    __shared__ float sf[(LHD_MAX_C + 1) * T::NS];
-    const int wi = blockIdx.x / T::TILES, tile = blockIdx.x % T::TILES;
-    const int tx0 = (tile % T::XT) * T::TX, trest = tile / T::XT;
-    const int ty0 = (trest % (H / T::TY)) * T::TY, tz0 = (trest / (H / T::TY)) * T::TZ;
-    const DevWorld &w = a.worlds[a.ids[wi]];
-    const int tid = threadIdx.x, C = w.C;
-    const int faces = (w.quirks & LHQ_FACE_HALO) != 0;
-    // Stage S (or SUM_S) and H_c (or U_S_c) with their halos.  Each slot's halo source
-    // is resolved once and shared by all fields.  The non-blur central-difference copy
-    // (reintegration.cpp:179-186) leaves U_S zero from padded index 5P/2 on.
-    const bool cdcopy = w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
-    const float *fbase[LHD_MAX_C + 1];
-    fbase[0] = (w.gradient == 0 && w.blur) ? a.pool + w.SUMS : a.pool + w.S;
-#pragma unroll
-    for (int c = 0; c < LHD_MAX_C; c++)
-        fbase[c + 1] = ((w.gradient == 0 && w.blur) ? a.pool + w.US : a.pool + w.Hg) + c * S::N;
-    for (int q = tid; q < T::NS; q += T::THREADS) {
-        const int sx = q % T::SX, sy = (q / T::SX) % T::SY, sz = q / (T::SX * T::SY);
-        int x = tx0 + sx - 1, y = ty0 + sy - 1, z = tz0 + sz - 1;
-        const bool ok = halo_src<W, H, D>(x, y, z, w.sobel_border, faces);
-        const int lin = x + W * (y + H * z);
-        const int pq = (int)S::pidx(x, y, z);
-#pragma unroll
-        for (int f = 0; f <= LHD_MAX_C; f++) {
-            if (f > C) break;
-            const bool cut = cdcopy && f > 0 && (long long)pq + (long long)(f - 1) * S::P >= 5 * S::P / 2;
-            sf[f * T::NS + q] = (ok && !cut) ? fbase[f][lin] : 0.0f;
-        }
-    }
-    __syncthreads();
+    const int C = w.C;
     const float *A = A_cur(a, w);
     const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
     const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
     const bool wall = w.border == 1;
     const bool qa = (w.quirks & LHQ_PREALPHA_INDEX) != 0;
-#pragma unroll
-    for (int j = 0; j < T::CPT; j++) {
-        const int cell = tid + j * T::THREADS;
-        if (cell >= T::NT) break;
-        const int tx = cell % T::TX, ty = (cell / T::TX) % T::TY, tz = cell / (T::TX * T::TY);
-        const int x = tx0 + tx, y = ty0 + ty, z = tz0 + tz;
-        const long long i = x + (long long)W * (y + (long long)H * z);
-        float na[3];
-        gradient_lds<T::SX, T::SY>(w, sf, tx, ty, tz, na);
-        const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
-        for (int c = 0; c < C; c++) {
-            float nu[3];
-            gradient_lds<T::SX, T::SY>(w, sf + (c + 1) * T::NS, tx, ty, tz, nu);
-            float al;
-            {
-                // PREALPHA written at c*P+i and read at c*N+i (lenia.cpp:660, 708)
-                const int r = c * S::N + (int)i;
-                int cs = c, is = (int)i;
-                bool ok = true;
-                if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = cs < C && is < S::N; }
-                if (ok) {
-                    float pg = A[cs * S::N + is] / w.theta_a;
-                    pg = pg * pg;
-                    al = clamp_ref(pg, 0.0f, 1.0f);
-                } else {
-                    al = 0.0f;
-                }
-            }
-            if (w.need_alpha) {
-                // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
-                const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
-                const int gc = r / S::N;
-                const float ga = (r < LHD_MAX_C * S::N && gc < C) ? a.pool[w.GA + r] : 0.0f;
-                a.pool[w.AL + c * S::N + i] = __builtin_fabsf(A[c * S::N + i] - ga);
-            }
-            const float al1 = 1.0f - al;
-            float F[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
-            float *V = a.pool + w.V + (long long)c * 3 * S::N;
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            if (w.v_live) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
-            }
-            if (w.mu_vel) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
-            }
-            float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
-            if (w.mode == 2) {
-                float vn[3];
-#pragma unroll
-                for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const float src = w.adv_vel ? vn[d] : F[d];
-                    MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
-                }
+    const long long i = x + (long long)W * (y + (long long)H * z);
+    float na[3];
+    grad(0, na);
+    const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+    for (int c = 0; c < C; c++) {
+        float nu[3];
+        grad(c + 1, nu);
+        float al;
+        {
+            // PREALPHA written at c*P+i and read at c*N+i (lenia.cpp:660, 708)
+            const int r = c * S::N + (int)i;
+            int cs = c, is = (int)i;
+            bool ok = true;
+            if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = cs < C && is < S::N; }
+            if (ok) {
+                float pg = A[cs * S::N + is] / w.theta_a;
+                pg = pg * pg;
+                al = clamp_ref(pg, 0.0f, 1.0f);
             } else {
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    float m = (w.mu_vel ? v[d] : F[d]) * dt;
-                    m = fmax_ref(m, nma);
-                    m = fmin_ref(m, ma);
-                    m = pos[d] + m;
-                    if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
-                    MU[(long long)d * S::N + i] = m;
-                }
+                al = 0.0f;
             }
-            if (!w.mu_vel) {
+        }
+        if (w.need_alpha) {
+            // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
+            const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
+            const int gc = r / S::N;
+            const float ga = (r < LHD_MAX_C * S::N && gc < C) ? a.pool[w.GA + r] : 0.0f;
+            a.pool[w.AL + c * S::N + i] = __builtin_fabsf(A[c * S::N + i] - ga);
+        }
+        const float al1 = 1.0f - al;
+        float F[3];
 #pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
-            }
-            if (w.v_live) {
+        for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
+        float *V = a.pool + w.V + (long long)c * 3 * S::N;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (w.v_live) {
 #pragma unroll
-                for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
-            }
-            if (w.need_f) {
-                float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
+            for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
+        }
+        if (w.mu_vel) {
 #pragma unroll
-                for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
+            for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
+        }
+        float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
+        if (w.mode == 2) {
+            float vn[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float src = w.adv_vel ? vn[d] : F[d];
+                MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
             }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                float m = (w.mu_vel ? v[d] : F[d]) * dt;
+                m = fmax_ref(m, nma);
+                m = fmin_ref(m, ma);
+                m = pos[d] + m;
+                if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
+                MU[(long long)d * S::N + i] = m;
+            }
+        }
+        if (!w.mu_vel) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
+        }
+        if (w.v_live) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
+        }
+        if (w.need_f) {
+            float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
+#pragma unroll
+            for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
         }
     }
 }
@@ -479,6 +401,153 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
     Sg[i] = w.stam * sold + ttsum * w.stas;
 }
 
+// Geometry of the z-sweep kernels (k_local_sweep, k_transport_sweep): a workgroup owns
+// a TX x TY column of cells through the whole depth and stages one z-plane at a time.
+template <int W, int H, int D, int CM>
+struct SweepCfg {
+    static constexpr int TX = W <= 48 ? W : 32;
+    static constexpr int XT = W / TX;
+    // 8 rows when the plane's weights for 4 channels fit 64 KB of LDS, else 4
+    static constexpr int TY = (TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));
+    static constexpr int YT = H / TY;
+    static constexpr int THREADS = TX * TY;
+    static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
+    static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
+    static constexpr int BLOCKS = XT * YT;
+    static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // arr, wx0..2, wy0..2, wz0..2
+    static_assert(LHD_MAX_C * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
+};
+
+// One coordinate through the transport halo (halo_src, axis by axis): resolved index
+// and whether it was outside [0, n).
+__device__ __forceinline__ int halo_axis(int v, int n, int kind, int &out) {
+    out = (v < 0) | (v >= n);
+    if (kind == 0) return v < 0 ? v + n : (v >= n ? v - n : v);
+    return v < 0 ? 0 : (v >= n ? n - 1 : v);
+}
+
+// Gradients from three staged z-planes (z-1, z, z+1).  Sobel taps in the reference order
+// dz, dy, dx (lenia.h:30-71); tap weights are +-1, +-2, +-4, so each product is exact and
+// fma(v, c, g) rounds exactly like the reference's separate multiply and add.  Central
+// differences as reintegration.cpp:260-315.
+template <int SX, bool PERM>
+__device__ __forceinline__ void sobel_planes(const float *const (&pl)[3], int tx, int ty, float (&g)[3]) {
+    constexpr int SW[3] = {1, 2, 1};
+    constexpr int SD[3] = {-1, 0, 1};
+    g[0] = 0.0f; g[1] = 0.0f; g[2] = 0.0f;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int gx = SD[dx + 1] * SW[dy + 1] * SW[dz + 1];
+                const int gy = SW[dx + 1] * SD[dy + 1] * SW[dz + 1];
+                const int gz = SW[dx + 1] * SW[dy + 1] * SD[dz + 1];
+                const int c0 = PERM ? gz : gx, c1 = PERM ? gx : gy, c2 = PERM ? gy : gz;
+                if (c0 == 0 && c1 == 0 && c2 == 0) continue;
+                const float v = pl[dz + 1][(tx + dx + 1) + SX * (ty + dy + 1)];
+                if (c0 != 0) g[0] = __builtin_fmaf(v, (float)c0, g[0]);
+                if (c1 != 0) g[1] = __builtin_fmaf(v, (float)c1, g[1]);
+                if (c2 != 0) g[2] = __builtin_fmaf(v, (float)c2, g[2]);
+            }
+}
+
+template <int SX>
+__device__ __forceinline__ void cdiff_planes(const float *const (&pl)[3], int tx, int ty, float (&g)[3]) {
+    auto at = [&](int x, int y, int zi) { return pl[zi][(x + 1) + SX * (y + 1)]; };
+    g[0] = (at(tx + 1, ty, 1) - at(tx - 1, ty, 1)) * 0.5f;
+    g[1] = (at(tx, ty + 1, 1) - at(tx, ty - 1, 1)) * 0.5f;
+    g[2] = (at(tx, ty, 2) - at(tx, ty, 0)) * 0.5f;
+}
+
+// Gradients, alpha, F, mu, V as a sweep along z: a workgroup owns a TX x TY column of
+// cells; S and H_c (one-cell x/y halo) are staged one z-plane per step into a ring of
+// three planes, the next plane already in flight, so every source value is read from
+// HBM once per column instead of once per tile it borders.
+template <int W, int H, int D, int CM>
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_sweep(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    using T = SweepCfg<W, H, D, CM>;
+    constexpr int F = CM + 1;  // staged fields: S, H_0 .. H_{C-1}
+    constexpr int LI = (F * T::SP + T::THREADS - 1) / T::THREADS;
+    __shared__ float ring[3 * F * T::SP];
+    const int wi = blockIdx.x / T::BLOCKS, b = blockIdx.x % T::BLOCKS;
+    const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    const int tid = threadIdx.x, C = w.C;
+    const int kind = w.sobel_border, faces = (w.quirks & LHQ_FACE_HALO) != 0;
+    // the non-blur central-difference copy (reintegration.cpp:179-186) leaves U_S zero
+    // from padded index 5P/2 on
+    const bool cdcopy = w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
+    const bool blur_src = w.gradient == 0 && w.blur;
+    const float *srcS = a.pool + (blur_src ? w.SUMS : w.S);
+    const float *srcH = a.pool + (blur_src ? w.US : w.Hg);
+    const int nitems = (C + 1) * T::SP;
+    int fld[LI], xy[LI], pqxy[LI];
+    bool ok_in[LI], ok_out[LI];
+#pragma unroll
+    for (int k = 0; k < LI; k++) {
+        const int item = tid + k * T::THREADS;
+        const int f = item / T::SP, q = item % T::SP;
+        int ox, oy;
+        const int xr = halo_axis(tx0 + q % T::SX - 1, W, kind, ox), yr = halo_axis(ty0 + q / T::SX - 1, H, kind, oy);
+        const int m = ox + oy;
+        const bool valid = item < nitems;
+        ok_in[k] = valid && (m == 0 || (kind != 2 && !(faces && m >= 2)));
+        ok_out[k] = valid && kind != 2 && !(faces && m + 1 >= 2);
+        fld[k] = f;
+        xy[k] = xr + W * yr;
+        pqxy[k] = (xr + PADR) + S::PW * (yr + PADR);
+    }
+    float pf[LI];
+    auto fetch = [&](int zp) {
+        int oz;
+        const int zr = halo_axis(zp, D, kind, oz);
+        const int zoff = W * H * zr, pqz = S::PW * S::PH * (zr + PADR);
+#pragma unroll
+        for (int k = 0; k < LI; k++) {
+            const int f = fld[k];
+            const bool cut = cdcopy && f > 0 && (long long)(pqxy[k] + pqz) + (long long)(f - 1) * S::P >= 5 * S::P / 2;
+            const bool ok = (oz ? ok_out[k] : ok_in[k]) && !cut;
+            const float *src = f == 0 ? srcS : srcH + (long long)(f - 1) * S::N;
+            pf[k] = ok ? src[xy[k] + zoff] : 0.0f;
+        }
+    };
+    auto store = [&](int zp) {
+        float *pl = ring + ((zp + 1) % 3) * F * T::SP;
+#pragma unroll
+        for (int k = 0; k < LI; k++) {
+            const int item = tid + k * T::THREADS;
+            if (item < nitems) pl[item] = pf[k];  // item = f * SP + q
+        }
+    };
+    fetch(-1);
+    store(-1);
+    fetch(0);
+    store(0);
+    fetch(1);
+    const int cx = tid % T::TX, cy = tid / T::TX;
+    const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
+    for (int z = 0; z < D; z++) {
+        __syncthreads();  // the plane slot about to be refilled is no longer read
+        store(z + 1);
+        if (z + 2 <= D) fetch(z + 2);
+        __syncthreads();
+        const float *p0 = ring + (z % 3) * F * T::SP, *p1 = ring + ((z + 1) % 3) * F * T::SP,
+                    *p2 = ring + ((z + 2) % 3) * F * T::SP;
+        local_cell<W, H, D>(a, w, tx0 + cx, ty0 + cy, z, [&](int f, float (&g)[3]) {
+            const float *const pl[3] = {p0 + f * T::SP, p1 + f * T::SP, p2 + f * T::SP};
+            if (w.gradient == 1) {
+                if (perm) sobel_planes<T::SX, true>(pl, cx, cy, g);
+                else sobel_planes<T::SX, false>(pl, cx, cy, g);
+            } else {
+                cdiff_planes<T::SX>(pl, cx, cy, g);
+            }
+        });
+    }
+}
+
 // trm_cell split in two so the sweep can load a cell's inputs a step before it needs
 // them: trm_load reads them, trm_store applies the same arithmetic as trm_cell.
 // The growth-after-transport inputs (H, A) are read at use; GA and S are prefetched.
@@ -536,29 +605,6 @@ __device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w,
 // complete after step z', when its TRM runs.  The product arr*wx*wy of a source is the
 // same for the three destinations and is formed once.
 // ------------------------------------------------------------------------------------
-template <int W, int H, int D, int CM>
-struct SweepCfg {
-    static constexpr int TX = W <= 48 ? W : 32;
-    static constexpr int XT = W / TX;
-    // 8 rows when the plane's weights for 4 channels fit 64 KB of LDS, else 4
-    static constexpr int TY = (TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));
-    static constexpr int YT = H / TY;
-    static constexpr int THREADS = TX * TY;
-    static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
-    static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
-    static constexpr int BLOCKS = XT * YT;
-    static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // arr, wx0..2, wy0..2, wz0..2
-    static_assert(LHD_MAX_C * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
-};
-
-// One coordinate through the transport halo (halo_src, axis by axis): resolved index
-// and whether it was outside [0, n).
-__device__ __forceinline__ int halo_axis(int v, int n, int kind, int &out) {
-    out = (v < 0) | (v >= n);
-    if (kind == 0) return v < 0 ? v + n : (v >= n ? v - n : v);
-    return v < 0 ? 0 : (v >= n ? n - 1 : v);
-}
-
 // CM: channels moved, at most (3 or LHD_MAX_C).
 template <int W, int H, int D, int CM>
 __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_transport_sweep(PhaseArgs a, int) {

@@ -233,7 +233,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "dft_iz", c.gops->iz, a, c.maxK);
             run(e, "dft_iy", c.gops->iy, a, c.maxK);
             run(e, "dft_ix_growth", c.gops->ix, a, 0);
-            run(e, "local", c.ops->local, a, 0);
+            run(e, "local", c.ops->local, a, c.maxC << 8);
             run(e, "transport", c.ops->transport, a, transport_arg(c));
             continue;
         }
@@ -244,7 +244,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "spec_z_blur", c.ops->spec_z, a, 1);
             run(e, "inv_blur", c.ops->inv_blur, a, 0);
         }
-        run(e, "local", c.ops->local, a, 0);
+        run(e, "local", c.ops->local, a, c.maxC << 8);
         run(e, "transport", c.ops->transport, a, transport_arg(c));
         PhaseArgs an = args_for(e, c, parity ^ 1);
         run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));

@@ -268,9 +268,17 @@ struct LaunchBig {
     static void inv_blur(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_inv_blur_big<W, H, D>), dim3(a.nworlds * D), dim3(Cfg::NT_INV), 0, (hipStream_t)st, a, 0);
     }
-    static void local(const PhaseArgs &a, int, void *st) {
-        hipLaunchKernelGGL((k_local_tiled<W, H, D>), dim3(a.nworlds * TileCfg<W, H, D>::TILES), dim3(256), 0,
-                           (hipStream_t)st, a, 0);
+    // arg: maxC << 8
+    static void local(const PhaseArgs &a, int arg, void *st) {
+        if ((arg >> 8) <= 3) {
+            using SC = SweepCfg<W, H, D, 3>;
+            hipLaunchKernelGGL((k_local_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS), 0,
+                               (hipStream_t)st, a, 0);
+        } else {
+            using SC = SweepCfg<W, H, D, LHD_MAX_C>;
+            hipLaunchKernelGGL((k_local_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS),
+                               0, (hipStream_t)st, a, 0);
+        }
     }
     // arg bits: 1 some world enables an oscillator, 2 some world is moved by the
     // z-sweep (RT, dd = 1, no oscillator), 4 some world is not; maxC << 8
