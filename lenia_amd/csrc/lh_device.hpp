@@ -403,12 +403,12 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
 
 // Geometry of the z-sweep kernels (k_local_sweep, k_transport_sweep): a workgroup owns
 // a TX x TY column of cells through the whole depth and stages one z-plane at a time.
-template <int W, int H, int D, int CM>
+template <int W, int H, int D, int CM, int TYMAX = 8>
 struct SweepCfg {
     static constexpr int TX = W <= 48 ? W : 32;
     static constexpr int XT = W / TX;
     // 8 rows when the plane's weights for 4 channels fit 64 KB of LDS, else 4
-    static constexpr int TY = (TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));
+    static constexpr int TY = (TYMAX >= 8 && TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));
     static constexpr int YT = H / TY;
     static constexpr int THREADS = TX * TY;
     static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
@@ -605,11 +605,13 @@ __device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w,
 // complete after step z', when its TRM runs.  The product arr*wx*wy of a source is the
 // same for the three destinations and is formed once.
 // ------------------------------------------------------------------------------------
-// CM: channels moved, at most (3 or LHD_MAX_C).
+// CM: channels moved, at most (3 or LHD_MAX_C).  Column height (rows): 8 measured
+// faster than 4 on the 32^3 batch.
+constexpr int TRANSPORT_TY = 8;
 template <int W, int H, int D, int CM>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_transport_sweep(PhaseArgs a, int) {
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) void k_transport_sweep(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
-    using T = SweepCfg<W, H, D, CM>;
+    using T = SweepCfg<W, H, D, CM, TRANSPORT_TY>;
     extern __shared__ float sw_dyn[];
     const int wi = blockIdx.x / T::BLOCKS, b = blockIdx.x % T::BLOCKS;
     const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;

@@ -409,11 +409,11 @@ struct Launch {
         if (arg & 2) {
             const int maxc = arg >> 8;
             if (maxc <= 3) {
-                using SC = SweepCfg<W, H, D, 3>;
+                using SC = SweepCfg<W, H, D, 3, TRANSPORT_TY>;
                 hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS),
                                    (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float), (hipStream_t)st, a, 0);
             } else {
-                using SC = SweepCfg<W, H, D, LHD_MAX_C>;
+                using SC = SweepCfg<W, H, D, LHD_MAX_C, TRANSPORT_TY>;
                 hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS),
                                    dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
                                    (hipStream_t)st, a, 0);
