@@ -401,6 +401,15 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
     Sg[i] = w.stam * sold + ttsum * w.stas;
 }
 
+// Planes per sweep workgroup: the whole depth when the batch alone gives >= 2048
+// workgroups, else segments of at least 4 planes until it does.
+inline int sweep_zseg(int nworlds, int blocks_per_world, int depth) {
+    const long long full = (long long)nworlds * blocks_per_world;
+    int nseg = full >= 2048 ? 1 : (int)((2048 + full - 1) / full);
+    if (nseg > depth / 4) nseg = depth / 4 > 0 ? depth / 4 : 1;
+    return (depth + nseg - 1) / nseg;
+}
+
 // Geometry of the z-sweep kernels (k_local_sweep, k_transport_sweep): a workgroup owns
 // a TX x TY column of cells through the whole depth and stages one z-plane at a time.
 template <int W, int H, int D, int CM, int TYMAX = 8>
@@ -465,14 +474,18 @@ __device__ __forceinline__ void cdiff_planes(const float *const (&pl)[3], int tx
 // cells; S and H_c (one-cell x/y halo) are staged one z-plane per step into a ring of
 // three planes, the next plane already in flight, so every source value is read from
 // HBM once per column instead of once per tile it borders.
+// zseg: planes per workgroup (the depth is split into segments when few worlds would
+// leave the GPU idle).
 template <int W, int H, int D, int CM>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_sweep(PhaseArgs a, int) {
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_sweep(PhaseArgs a, int zseg) {
     using S = Shape<W, H, D>;
     using T = SweepCfg<W, H, D, CM>;
     constexpr int F = CM + 1;  // staged fields: S, H_0 .. H_{C-1}
     constexpr int LI = (F * T::SP + T::THREADS - 1) / T::THREADS;
     __shared__ float ring[3 * F * T::SP];
-    const int wi = blockIdx.x / T::BLOCKS, b = blockIdx.x % T::BLOCKS;
+    const int nseg = (D + zseg - 1) / zseg;
+    const int z0 = (blockIdx.x % nseg) * zseg, z1 = z0 + zseg < D ? z0 + zseg : D;
+    const int wi = blockIdx.x / nseg / T::BLOCKS, b = (blockIdx.x / nseg) % T::BLOCKS;
     const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C;
@@ -522,17 +535,17 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
             if (item < nitems) pl[item] = pf[k];  // item = f * SP + q
         }
     };
-    fetch(-1);
-    store(-1);
-    fetch(0);
-    store(0);
-    fetch(1);
+    fetch(z0 - 1);
+    store(z0 - 1);
+    fetch(z0);
+    store(z0);
+    fetch(z0 + 1);
     const int cx = tid % T::TX, cy = tid / T::TX;
     const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
-    for (int z = 0; z < D; z++) {
+    for (int z = z0; z < z1; z++) {
         __syncthreads();  // the plane slot about to be refilled is no longer read
         store(z + 1);
-        if (z + 2 <= D) fetch(z + 2);
+        if (z + 2 <= z1) fetch(z + 2);
         __syncthreads();
         const float *p0 = ring + (z % 3) * F * T::SP, *p1 = ring + ((z + 1) % 3) * F * T::SP,
                     *p2 = ring + ((z + 2) % 3) * F * T::SP;
@@ -609,11 +622,15 @@ __device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w,
 // faster than 4 on the 32^3 batch.
 constexpr int TRANSPORT_TY = 8;
 template <int W, int H, int D, int CM>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) void k_transport_sweep(PhaseArgs a, int) {
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) void k_transport_sweep(PhaseArgs a,
+                                                                                                   int zseg) {
     using S = Shape<W, H, D>;
     using T = SweepCfg<W, H, D, CM, TRANSPORT_TY>;
     extern __shared__ float sw_dyn[];
-    const int wi = blockIdx.x / T::BLOCKS, b = blockIdx.x % T::BLOCKS;
+    // destination planes [z0, z1) of this workgroup; source planes z0-1 .. z1
+    const int nseg = (D + zseg - 1) / zseg;
+    const int z0 = (blockIdx.x % nseg) * zseg, z1 = z0 + zseg < D ? z0 + zseg : D;
+    const int wi = blockIdx.x / nseg / T::BLOCKS, b = (blockIdx.x / nseg) % T::BLOCKS;
     const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;
     const DevWorld &w = a.worlds[a.ids[wi]];
     if (!sweep_world(w)) return;
@@ -673,7 +690,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
     }
     TrmIn<CM> pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
     auto step = [&](int zp, const float (&cur)[T::ITEMS][4], float (&nxt)[T::ITEMS][4]) {
-        if (zp < D) fetch(zp + 1, nxt);
+        if (zp < z1) fetch(zp + 1, nxt);
         __syncthreads();  // the previous plane's weights are no longer read
         const float shz = !torus ? 0.0f : (zp < 0 ? (float)D : (zp >= D ? -(float)D : 0.0f));
         const float posz = (float)zp + 0.5f;
@@ -726,8 +743,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
                 }
         }
         const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
-        if (zp >= 1) trm_store<W, H, D, CM>(a, w, col + (long long)W * H * (zp - 1), accP, pre);
-        if (zp >= 0 && zp < D) trm_load<W, H, D, CM>(a, w, col + (long long)W * H * zp, pre);  // used next step
+        if (zp >= z0 + 1) trm_store<W, H, D, CM>(a, w, col + (long long)W * H * (zp - 1), accP, pre);
+        if (zp >= z0 && zp < z1) trm_load<W, H, D, CM>(a, w, col + (long long)W * H * zp, pre);  // used next step
 #pragma unroll
         for (int c = 0; c < CM; c++) {
             accP[c] = accC[c];
@@ -736,8 +753,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
         }
     };
     float cur[T::ITEMS][4], nxt[T::ITEMS][4];
-    fetch(-1, cur);
-    for (int zp = -1; zp <= D; zp++) {
+    fetch(z0 - 1, cur);
+    for (int zp = z0 - 1; zp <= z1; zp++) {
         step(zp, cur, nxt);
 #pragma unroll
         for (int k = 0; k < T::ITEMS; k++)

@@ -395,12 +395,14 @@ struct Launch {
     static void local(const PhaseArgs &a, int arg, void *st) {
         if ((arg >> 8) <= 3) {
             using SC = SweepCfg<W, H, D, 3>;
-            hipLaunchKernelGGL((k_local_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS), 0,
-                               (hipStream_t)st, a, 0);
+            const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+            hipLaunchKernelGGL((k_local_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
+                               (hipStream_t)st, a, zs);
         } else {
             using SC = SweepCfg<W, H, D, LHD_MAX_C>;
-            hipLaunchKernelGGL((k_local_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS),
-                               0, (hipStream_t)st, a, 0);
+            const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+            hipLaunchKernelGGL((k_local_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
+                               dim3(SC::THREADS), 0, (hipStream_t)st, a, zs);
         }
     }
     // arg bits: 1 some world enables an oscillator, 2 some world is moved by the
@@ -410,13 +412,16 @@ struct Launch {
             const int maxc = arg >> 8;
             if (maxc <= 3) {
                 using SC = SweepCfg<W, H, D, 3, TRANSPORT_TY>;
-                hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS), dim3(SC::THREADS),
-                                   (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float), (hipStream_t)st, a, 0);
+                const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+                hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns),
+                                   dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
+                                   (hipStream_t)st, a, zs);
             } else {
                 using SC = SweepCfg<W, H, D, LHD_MAX_C, TRANSPORT_TY>;
-                hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS),
+                const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+                hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
                                    dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
-                                   (hipStream_t)st, a, 0);
+                                   (hipStream_t)st, a, zs);
             }
         }
         if (arg & 4) {
