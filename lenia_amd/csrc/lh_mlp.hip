@@ -160,22 +160,33 @@ __global__ void k_loss(const float *out, const float *y, int M, int nvalid, floa
         g[2] = 2.0f * d2;
     }
     rl[r] = l;
-    for (int c = 0; c < OUTP; c++) {
-        const float v = c < 3 ? g[c] * scale : 0.0f;
+    // columns 3.. stay zero from lm_create
+    for (int c = 0; c < 3; c++) {
+        const float v = g[c] * scale;
         dout[(size_t)r * OUTP + c] = v;
         dout16[(size_t)r * OUTP + c] = to_bf16(v);
     }
 }
 
-// Column sums of dZ [M][N] fp32 -> db[N] (one block of 256 threads per 64 columns).
-__global__ void k_colsum(const float *dz, int M, int N, float *db) {
-    __shared__ float red[4][64];
+// Column sums of dZ [M][N] fp32 -> db[N]: one block of 1024 threads per 64 columns, 16
+// row phases per column, partial sums combined in a fixed order.
+__global__ __launch_bounds__(1024) void k_colsum(const float *dz, int M, int N, float *db) {
+    __shared__ float red[16][64];
     const int c = blockIdx.x * 64 + (threadIdx.x & 63), part = threadIdx.x >> 6;
-    float s = 0.0f;
-    for (int r = part; r < M; r += 4) s += dz[(size_t)r * N + c];
-    red[part][threadIdx.x & 63] = s;
+    float s0 = 0.0f, s1 = 0.0f;
+    int r = part;
+    for (; r + 16 < M; r += 32) {
+        s0 += dz[(size_t)r * N + c];
+        s1 += dz[(size_t)(r + 16) * N + c];
+    }
+    if (r < M) s0 += dz[(size_t)r * N + c];
+    red[part][threadIdx.x & 63] = s0 + s1;
     __syncthreads();
-    if (part == 0) db[c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    if (part == 0) {
+        float t = 0.0f;
+        for (int k = 0; k < 16; k++) t += red[k][threadIdx.x];
+        db[c] = t;
+    }
 }
 
 // Sum of row losses -> *out (single block).
@@ -323,7 +334,7 @@ void backward(Mlp &n, int M, int nvalid) {
     // layer 4: dW4 = H3^T dO, db4, dH3 = dO W4^T -> dZ3
     a.A = n.H3; a.B = n.dO16; a.M = HID; a.N = OUTP; a.K = M; a.lda = HID; a.ldb = OUTP; a.ldc = OUTP; a.C = n.g + OW4;
     gemm<true, false, EPI_F32>(a, n.st);
-    hipLaunchKernelGGL(k_colsum, dim3(OUTP / 64), dim3(256), 0, n.st, n.dO, M, OUTP, n.g + OB4);
+    hipLaunchKernelGGL(k_colsum, dim3(OUTP / 64), dim3(1024), 0, n.st, n.dO, M, OUTP, n.g + OB4);
     a = GemmArgs{};
     a.A = n.dO16; a.B = n.p16 + OW4; a.M = M; a.N = HID; a.K = OUTP; a.lda = OUTP; a.ldb = OUTP; a.ldc = HID;
     a.Z = n.Z3; a.C = n.dZa; a.H = n.dA16;
@@ -339,7 +350,7 @@ void backward(Mlp &n, int M, int nvalid) {
         a = GemmArgs{};
         a.A = Hs[l]; a.B = dz16; a.M = kin; a.N = HID; a.K = M; a.lda = kin; a.ldb = HID; a.ldc = HID; a.C = n.g + OW[l];
         gemm<true, false, EPI_F32>(a, n.st);
-        hipLaunchKernelGGL(k_colsum, dim3(HID / 64), dim3(256), 0, n.st, dz, M, HID, n.g + OB[l]);
+        hipLaunchKernelGGL(k_colsum, dim3(HID / 64), dim3(1024), 0, n.st, dz, M, HID, n.g + OB[l]);
         if (l == 0) break;
         a = GemmArgs{};
         a.A = dz16; a.B = n.p16 + OW[l]; a.M = M; a.N = HID; a.K = HID; a.lda = HID; a.ldb = HID; a.ldc = HID;
@@ -383,6 +394,7 @@ lh_status lm_create(int device, int max_batch, void **out) {
     MCHK(hipMalloc(&n->dA16, c * HID * 2));
     MCHK(hipMalloc(&n->dB16, c * HID * 2));
     MCHK(hipMalloc(&n->dO16, c * OUTP * 2));
+    MCHK(hipMemset(n->dO16, 0, c * OUTP * 2));
     MCHK(hipMalloc(&n->Z1, c * HID * 4));
     MCHK(hipMalloc(&n->Z2, c * HID * 4));
     MCHK(hipMalloc(&n->Z3, c * HID * 4));
@@ -390,6 +402,7 @@ lh_status lm_create(int device, int max_batch, void **out) {
     MCHK(hipMalloc(&n->dZb, c * HID * 4));
     MCHK(hipMalloc(&n->O, c * OUTP * 4));
     MCHK(hipMalloc(&n->dO, c * OUTP * 4));
+    MCHK(hipMemset(n->dO, 0, c * OUTP * 4));
     MCHK(hipMalloc(&n->Y, c * 3 * 4));
     MCHK(hipMalloc(&n->rl, c * 4));
     MCHK(hipMalloc(&n->lsum, 4));
