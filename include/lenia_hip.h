@@ -96,7 +96,10 @@ enum {
     LH_OPT_KEEP_VELOCITY = 0, /* maintain V even where the mode never reads it (default 0) */
     LH_OPT_KEEP_USUM = 1,     /* write U_SUM for mesh views (default 0) */
     LH_OPT_USE_GRAPH = 2,     /* replay steps from a captured hipGraph (default 1) */
-    LH_OPT_PROFILE = 3        /* time every kernel launch with HIP events (default 0) */
+    LH_OPT_PROFILE = 3,       /* time every kernel launch with HIP events (default 0) */
+    LH_OPT_AUTOTUNE = 4       /* lh_commit times the candidate schedules of each shape class
+                                 (sweep segment lengths, sweep vs tiled transport) on the
+                                 class's worlds and keeps the fastest (default 1) */
 };
 
 /* Growth convolution of a world (lh_set_world_conv).  LH_CONV_FFT is the fp32 FFT

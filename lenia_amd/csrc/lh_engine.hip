@@ -135,6 +135,9 @@ struct ShapeClass {
     bool any_blur = false;
     bool any_osc = false;
     bool any_sweep = false, any_other = false;  // transport by the z-sweep / by tiles
+    // schedule (lh_commit's autotuner; 0 / false = the launchers' defaults)
+    int zseg_local = 0, zseg_transport = 0;
+    bool tiled_transport = false;  // the tiled kernel also moves the sweep's worlds
     int conv = LH_CONV_FFT;          // growth convolution of every world in the class
     const GemmOps *gops = nullptr;   // matrix-core passes when conv != LH_CONV_FFT
     unsigned short *d_dft = nullptr; // their DFT matrices
@@ -163,7 +166,7 @@ struct Engine {
     size_t mc_cap = 0;
     int parity = 0;
     bool spectra_dirty = true;
-    int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0;
+    int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0, opt_autotune = 1;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     std::vector<std::string> prof_names;
     std::map<std::string, Prof> prof;
@@ -212,8 +215,11 @@ void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg)
 }
 
 int transport_arg(const ShapeClass &c) {
-    return (c.any_osc ? 1 : 0) | (c.any_sweep ? 2 : 0) | (c.any_other ? 4 : 0) | (c.maxC << 8);
+    const bool sweep = c.any_sweep && !c.tiled_transport, tiles = c.any_other || (c.any_sweep && c.tiled_transport);
+    return (c.any_osc ? 1 : 0) | (sweep ? 2 : 0) | (tiles ? 4 : 0) | (c.maxC << 8) | (c.zseg_transport << 16);
 }
+
+int local_arg(const ShapeClass &c) { return (c.maxC << 8) | (c.zseg_local << 16); }
 
 void enqueue_spectra(Engine &e) {
     for (ShapeClass &c : e.classes) {
@@ -233,7 +239,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "dft_iz", c.gops->iz, a, c.maxK);
             run(e, "dft_iy", c.gops->iy, a, c.maxK);
             run(e, "dft_ix_growth", c.gops->ix, a, 0);
-            run(e, "local", c.ops->local, a, c.maxC << 8);
+            run(e, "local", c.ops->local, a, local_arg(c));
             run(e, "transport", c.ops->transport, a, transport_arg(c));
             continue;
         }
@@ -244,7 +250,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "spec_z_blur", c.ops->spec_z, a, 1);
             run(e, "inv_blur", c.ops->inv_blur, a, 0);
         }
-        run(e, "local", c.ops->local, a, c.maxC << 8);
+        run(e, "local", c.ops->local, a, local_arg(c));
         run(e, "transport", c.ops->transport, a, transport_arg(c));
         PhaseArgs an = args_for(e, c, parity ^ 1);
         run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));
@@ -354,6 +360,65 @@ int64_t field_off(const Engine &e, const HostWorld &w, int field) {
     case LH_F_USUM: return d.USUM;
     }
     return -1;
+}
+
+// Average time of reps launches of one phase (after one untimed launch).
+float time_phase(Engine &e, launch_fn fn, const PhaseArgs &a, int arg, int reps) {
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    fn(a, arg, e.stream);
+    hipEventRecord(t0, e.stream);
+    for (int r = 0; r < reps; r++) fn(a, arg, e.stream);
+    hipEventRecord(t1, e.stream);
+    hipEventSynchronize(t1);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return ms / reps;
+}
+
+// Schedule autotuner: per shape class, times the gradient sweep at several segment
+// lengths and the transport as z-sweep (same lengths) or as tiles, on the class's own
+// worlds, and keeps the fastest.  Runs on the freshly zeroed pool (every kernel is
+// defined on zeros); the caller clears the pool afterwards.
+lh_status autotune(Engine &e) {
+    for (ShapeClass &c : e.classes) {
+        PhaseArgs a = args_for(e, c, 0);
+        const int D = c.D;
+        std::vector<int> cands = {D};
+        for (int zs : {D / 2, D / 4, 8, 4})
+            if (zs >= 4 && std::find(cands.begin(), cands.end(), zs) == cands.end()) cands.push_back(zs);
+        const int reps = c.ids.size() >= 256 ? 2 : 5;
+        float best = 1e30f;
+        int pick_local = D;
+        for (int zs : cands) {
+            c.zseg_local = zs;
+            const float t = time_phase(e, c.ops->local, a, local_arg(c), reps);
+            if (t < best) { best = t; pick_local = zs; }
+        }
+        c.zseg_local = pick_local;
+        if (c.any_sweep) {
+            best = 1e30f;
+            int pick = 0;
+            bool tiled = false;
+            for (int zs : cands) {
+                c.zseg_transport = zs;
+                c.tiled_transport = false;
+                const float t = time_phase(e, c.ops->transport, a, transport_arg(c), reps);
+                if (t < best) { best = t; pick = zs; tiled = false; }
+            }
+            c.tiled_transport = true;
+            c.zseg_transport = 0;
+            const float t = time_phase(e, c.ops->transport, a, transport_arg(c), reps);
+            if (t < best) { best = t; tiled = true; pick = 0; }
+            c.tiled_transport = tiled;
+            c.zseg_transport = pick;
+        }
+        HIPCHK(hipGetLastError());
+    }
+    return LH_OK;
 }
 
 lh_status build_kernel_spectra(Engine &e) {
@@ -491,6 +556,10 @@ lh_status lh_set_option(void *p, int option, int value) {
         if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
         (option == LH_OPT_KEEP_VELOCITY ? e->opt_keep_v : e->opt_keep_usum) = value != 0;
         return LH_OK;
+    case LH_OPT_AUTOTUNE:
+        if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
+        e->opt_autotune = value != 0;
+        return LH_OK;
     case LH_OPT_USE_GRAPH: e->opt_graph = value != 0; return LH_OK;
     case LH_OPT_PROFILE: e->opt_profile = value != 0; return LH_OK;
     }
@@ -621,6 +690,11 @@ lh_status lh_commit(void *p) {
         }
     }
     HIPCHK(hipMalloc(&e->d_mass, e->worlds.size() * sizeof(double)));
+    if (e->opt_autotune) {
+        lh_status t = autotune(*e);
+        if (t != LH_OK) return t;
+        HIPCHK(hipMemsetAsync(e->pool, 0, e->pool_floats * sizeof(float), e->stream));
+    }
     lh_status s = build_kernel_spectra(*e);
     if (s != LH_OK) return s;
     e->committed = true;

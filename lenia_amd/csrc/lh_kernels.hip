@@ -177,6 +177,27 @@ __global__ __launch_bounds__(128) void k_kspec_z(PhaseArgs a, int) {
 // c2r along x, then per cell in kernel order  u = g/N; U_SUM += u; H += bell(u)*h
 // (mt_fft_step lenia.cpp:496-538).  Finally B = clamp(A + H*dt) (lenia.cpp:542-592).
 // ------------------------------------------------------------------------------------
+// LDS <- nk spectrum planes (PLANE float2 each, plane kk at src + kk * stride) by 16-byte
+// direct-to-LDS loads: each wave instruction moves 1 KB and holds no registers, so a
+// workgroup's whole staging is in flight at once.  Plane sizes are multiples of 16 bytes
+// (H even) and plane starts 16-byte aligned; the next barrier waits for the loads.
+template <int PLANE, int T>
+__device__ __forceinline__ void dma_planes(float2 *dst, const float2 *src, long long stride, int nk, int tid) {
+    constexpr int BYTES = PLANE * 8, NCH = (BYTES + 1023) / 1024, WAVES = T / 64;
+    static_assert(BYTES % 16 == 0, "plane size");
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int t = wave; t < nk * NCH; t += WAVES) {
+        const int kk = t / NCH, c = t % NCH;
+        const int off = c * 1024 + lane * 16;
+        if (off < BYTES) {
+            const char *g = reinterpret_cast<const char *>(src + kk * stride) + off;
+            char *l = reinterpret_cast<char *>(dst + kk * PLANE) + c * 1024;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                             (__attribute__((address_space(3))) void *)l, 16, 0, 0);
+        }
+    }
+}
+
 template <int W, int H, int D>
 struct InvCfg {
     static constexpr int PLANE = H * (W / 2 + 1);  // float2 per spectrum plane
@@ -188,13 +209,13 @@ struct InvCfg {
 
 // CM: channels per world at most (3 or 4); KU: U_SUM is written.
 template <int W, int H, int D, int CM, bool KU>
-__global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
+__global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using Cfg = InvCfg<W, H, D>;
     constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
     constexpr int CPT = (PL + T - 1) / T;  // cells of the plane owned by each thread
     constexpr int CU = KU ? CM : 1;        // U_SUM accumulators
-    __shared__ float2 sg[KB * Cfg::PLANE];
+    __shared__ __attribute__((aligned(16))) float2 sg[KB * Cfg::PLANE];
     const int wi = blockIdx.x / D, z = blockIdx.x % D;
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C, K = w.K;
@@ -217,10 +238,7 @@ __global__ __launch_bounds__(256) void k_inv_growth(PhaseArgs a, int) {
     for (int k0 = 0; k0 < K; k0 += KB) {
         const int nk = K - k0 < KB ? K - k0 : KB;
         __syncthreads();
-        for (int i = tid; i < nk * Cfg::PLANE; i += T) {
-            const int kk = i / Cfg::PLANE, j = i % Cfg::PLANE;
-            sg[i] = G[(long long)(k0 + kk) * S::Nh + (long long)z * Cfg::PLANE + j];
-        }
+        dma_planes<Cfg::PLANE, T>(sg, G + (long long)k0 * S::Nh + (long long)z * Cfg::PLANE, S::Nh, nk, tid);
         __syncthreads();
         for (int t = tid; t < nk * Wh; t += T) {
             const int kk = t / Wh, kx = t % Wh;
@@ -308,7 +326,7 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using Cfg = InvCfg<W, H, D>;
     constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
-    __shared__ float2 sg[KB * Cfg::PLANE];
+    __shared__ __attribute__((aligned(16))) float2 sg[KB * Cfg::PLANE];
     const int wi = blockIdx.x / D, z = blockIdx.x % D;
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, NF = w.C + 1;
@@ -317,10 +335,7 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
     for (int f0 = 0; f0 < NF; f0 += KB) {
         const int nk = NF - f0 < KB ? NF - f0 : KB;
         __syncthreads();
-        for (int i = tid; i < nk * Cfg::PLANE; i += T) {
-            const int kk = i / Cfg::PLANE, j = i % Cfg::PLANE;
-            sg[i] = G[(long long)(f0 + kk) * S::Nh + (long long)z * Cfg::PLANE + j];
-        }
+        dma_planes<Cfg::PLANE, T>(sg, G + (long long)f0 * S::Nh + (long long)z * Cfg::PLANE, S::Nh, nk, tid);
         __syncthreads();
         for (int t = tid; t < nk * Wh; t += T) {
             const int kk = t / Wh, kx = t % Wh;
@@ -391,34 +406,35 @@ struct Launch {
     static void inv_blur(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_inv_blur<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
     }
-    // arg: maxC << 8
+    // arg: maxC << 8 | planes per sweep workgroup << 16 (0: sweep_zseg)
     static void local(const PhaseArgs &a, int arg, void *st) {
-        if ((arg >> 8) <= 3) {
+        if (((arg >> 8) & 0xff) <= 3) {
             using SC = SweepCfg<W, H, D, 3>;
-            const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+            const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
             hipLaunchKernelGGL((k_local_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
                                (hipStream_t)st, a, zs);
         } else {
             using SC = SweepCfg<W, H, D, LHD_MAX_C>;
-            const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+            const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
             hipLaunchKernelGGL((k_local_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
                                dim3(SC::THREADS), 0, (hipStream_t)st, a, zs);
         }
     }
     // arg bits: 1 some world enables an oscillator, 2 some world is moved by the
-    // z-sweep (RT, dd = 1, no oscillator), 4 some world is not; maxC << 8
+    // z-sweep (RT, dd = 1, no oscillator), 4 some world is not (or the tiled kernel moves
+    // every world); maxC << 8; planes per sweep workgroup << 16 (0: sweep_zseg)
     static void transport(const PhaseArgs &a, int arg, void *st) {
         if (arg & 2) {
-            const int maxc = arg >> 8;
+            const int maxc = (arg >> 8) & 0xff;
             if (maxc <= 3) {
                 using SC = SweepCfg<W, H, D, 3, TRANSPORT_TY>;
-                const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+                const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
                 hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns),
                                    dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
                                    (hipStream_t)st, a, zs);
             } else {
                 using SC = SweepCfg<W, H, D, LHD_MAX_C, TRANSPORT_TY>;
-                const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+                const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
                 hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
                                    dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
                                    (hipStream_t)st, a, zs);

@@ -268,34 +268,35 @@ struct LaunchBig {
     static void inv_blur(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_inv_blur_big<W, H, D>), dim3(a.nworlds * D), dim3(Cfg::NT_INV), 0, (hipStream_t)st, a, 0);
     }
-    // arg: maxC << 8
+    // arg: maxC << 8 | planes per sweep workgroup << 16 (0: sweep_zseg)
     static void local(const PhaseArgs &a, int arg, void *st) {
-        if ((arg >> 8) <= 3) {
+        if (((arg >> 8) & 0xff) <= 3) {
             using SC = SweepCfg<W, H, D, 3>;
-            const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+            const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
             hipLaunchKernelGGL((k_local_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
                                (hipStream_t)st, a, zs);
         } else {
             using SC = SweepCfg<W, H, D, LHD_MAX_C>;
-            const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+            const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
             hipLaunchKernelGGL((k_local_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
                                dim3(SC::THREADS), 0, (hipStream_t)st, a, zs);
         }
     }
     // arg bits: 1 some world enables an oscillator, 2 some world is moved by the
-    // z-sweep (RT, dd = 1, no oscillator), 4 some world is not; maxC << 8
+    // z-sweep (RT, dd = 1, no oscillator), 4 some world is not (or the tiled kernel moves
+    // every world); maxC << 8; planes per sweep workgroup << 16 (0: sweep_zseg)
     static void transport(const PhaseArgs &a, int arg, void *st) {
         if (arg & 2) {
-            const int maxc = arg >> 8;
+            const int maxc = (arg >> 8) & 0xff;
             if (maxc <= 3) {
                 using SC = SweepCfg<W, H, D, 3, TRANSPORT_TY>;
-                const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+                const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
                 hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns),
                                    dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
                                    (hipStream_t)st, a, zs);
             } else {
                 using SC = SweepCfg<W, H, D, LHD_MAX_C, TRANSPORT_TY>;
-                const int zs = sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+                const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
                 hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
                                    dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
                                    (hipStream_t)st, a, zs);

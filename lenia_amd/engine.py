@@ -19,7 +19,7 @@ class Engine:
               "MU": L.F_MU, "USUM": L.F_USUM}
 
     def __init__(self, device: int = 0, keep_velocity=False, keep_usum=False, graph=True,
-                 profile=False):
+                 profile=False, autotune=True):
         self._L = L.load()
         h = C.c_void_p()
         L.check(self._L.lh_create(device, C.byref(h)), "lh_create")
@@ -29,6 +29,7 @@ class Engine:
         L.check(self._L.lh_set_option(self._h, L.OPT_KEEP_USUM, int(keep_usum)), "keep_usum")
         self.set_option(L.OPT_USE_GRAPH, graph)
         self.set_option(L.OPT_PROFILE, profile)
+        self.set_option(L.OPT_AUTOTUNE, autotune)
 
     def set_option(self, opt, value):
         L.check(self._L.lh_set_option(self._h, opt, int(value)), "lh_set_option")

@@ -257,3 +257,23 @@ def test_rejects_bad_input():
     with pytest.raises(L.LeniaHipError, match="U_SUM"):
         eng.get(wid, "USUM")
     eng.close()
+
+
+def test_autotuned_and_default_schedules_identical():
+    """The autotuner only picks among schedules with the same per-cell arithmetic: a
+    mixed batch steps bit-identically with and without it."""
+    from lenia_amd.engine import Engine
+    ws = WL.mixed(n_worlds=8, base_seed=500, asymptotic=ASYM)
+    outs = []
+    for tune in (True, False):
+        eng = Engine(autotune=tune)
+        for w in ws:
+            eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        for i, w in enumerate(ws):
+            eng.set(i, "A", w["A"])
+        eng.step(5)
+        outs.append([eng.get(i, "A") for i in range(len(ws))])
+        eng.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
