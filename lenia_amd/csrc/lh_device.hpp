@@ -126,11 +126,38 @@ struct TileCfg {
     __device__ static __forceinline__ int slot(int tx, int ty, int tz) { return tx + SX * (ty + SY * tz); }
 };
 
+// The A value alpha reads per channel (PREALPHA index quirk applied; 0 when the quirk's
+// index falls outside the channels), loaded by the sweep one plane ahead of its use.
+template <int CM>
+struct LocalIn {
+    float aal[CM];
+};
+
+template <int W, int H, int D, int CM>
+__device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w, long long i, LocalIn<CM> &in) {
+    using S = Shape<W, H, D>;
+    const int C = w.C;
+    const float *A = A_cur(a, w);
+    const bool qa = (w.quirks & LHQ_PREALPHA_INDEX) != 0;
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        // PREALPHA written at c*P+i and read at c*N+i (lenia.cpp:660, 708)
+        const int r = c * S::N + (int)i;
+        int cs = c, is = (int)i;
+        bool ok = c < C;
+        if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = ok && cs < C && is < S::N; }
+        in.aal[c] = ok ? A[cs * S::N + is] : 0.0f;
+    }
+}
+
 // Gradients -> alpha, F, mu, V for one cell (sobel_convolve_h lenia.cpp:647-672,
 // compute_f lenia.cpp:679-719, compute_*_mu reintegration.cpp:321-602); grad(f, g)
-// returns the gradient of staged field f (0: S or SUM_S, 1 + c: H_c or U_S_c).
-template <int W, int H, int D, class Grad>
-__device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z, Grad grad) {
+// returns the gradient of staged field f (0: S or SUM_S, 1 + c: H_c or U_S_c).  A
+// quirk-index alpha source outside the channels gives alpha = 0, which equals
+// clamp((0 / theta)^2, 0, 1).
+template <int W, int H, int D, int CM, class Grad>
+__device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
+                                           const LocalIn<CM> &in, Grad grad) {
     using S = Shape<W, H, D>;
     const int C = w.C;
     const float *A = A_cur(a, w);
@@ -142,24 +169,14 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
     float na[3];
     grad(0, na);
     const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
-    for (int c = 0; c < C; c++) {
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        if (c >= C) break;
         float nu[3];
         grad(c + 1, nu);
-        float al;
-        {
-            // PREALPHA written at c*P+i and read at c*N+i (lenia.cpp:660, 708)
-            const int r = c * S::N + (int)i;
-            int cs = c, is = (int)i;
-            bool ok = true;
-            if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = cs < C && is < S::N; }
-            if (ok) {
-                float pg = A[cs * S::N + is] / w.theta_a;
-                pg = pg * pg;
-                al = clamp_ref(pg, 0.0f, 1.0f);
-            } else {
-                al = 0.0f;
-            }
-        }
+        float pg = in.aal[c] / w.theta_a;
+        pg = pg * pg;
+        const float al = clamp_ref(pg, 0.0f, 1.0f);
         if (w.need_alpha) {
             // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
             const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
@@ -172,11 +189,9 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
 #pragma unroll
         for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
         float *V = a.pool + w.V + (long long)c * 3 * S::N;
-        float v[3] = {0.0f, 0.0f, 0.0f};
-        if (w.v_live) {
+        float v[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) v[d] = V[(long long)d * S::N + i];
-        }
+        for (int d = 0; d < 3; d++) v[d] = w.v_live ? V[(long long)d * S::N + i] : 0.0f;
         if (w.mu_vel) {
 #pragma unroll
             for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
@@ -542,6 +557,9 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     fetch(z0 + 1);
     const int cx = tid % T::TX, cy = tid / T::TX;
     const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
+    const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
+    LocalIn<CM> in;
+    local_load<W, H, D, CM>(a, w, col + (long long)W * H * z0, in);
     for (int z = z0; z < z1; z++) {
         __syncthreads();  // the plane slot about to be refilled is no longer read
         store(z + 1);
@@ -549,7 +567,9 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
         __syncthreads();
         const float *p0 = ring + (z % 3) * F * T::SP, *p1 = ring + ((z + 1) % 3) * F * T::SP,
                     *p2 = ring + ((z + 2) % 3) * F * T::SP;
-        local_cell<W, H, D>(a, w, tx0 + cx, ty0 + cy, z, [&](int f, float (&g)[3]) {
+        const LocalIn<CM> cur = in;
+        if (z + 1 < z1) local_load<W, H, D, CM>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
+        local_cell<W, H, D, CM>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
             const float *const pl[3] = {p0 + f * T::SP, p1 + f * T::SP, p2 + f * T::SP};
             if (w.gradient == 1) {
                 if (perm) sobel_planes<T::SX, true>(pl, cx, cy, g);
