@@ -761,6 +761,10 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
                     t = t * sn;
                     accP[c] = accP[c] + t;
                 }
+            // this channel's sums are complete here: the compiler may not sink their
+            // arithmetic below the next channel's LDS reads (which would keep two
+            // channels' staged values live at once)
+            asm volatile("" : "+v"(accN[c]), "+v"(accC[c]), "+v"(accP[c]));
         }
         const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
         if (zp >= z0 + 1) trm_store<W, H, D, CM>(a, w, col + (long long)W * H * (zp - 1), accP, pre);
