@@ -172,6 +172,9 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
 #pragma unroll
     for (int c = 0; c < CM; c++) {
         if (c >= C) break;
+        // one channel's staged values live at a time: no LDS read of this channel is
+        // hoisted above the previous channel's stores
+        asm volatile("" ::: "memory");
         float nu[3];
         grad(c + 1, nu);
         float pg = in.aal[c] / w.theta_a;
@@ -502,7 +505,9 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     const int z0 = (blockIdx.x % nseg) * zseg, z1 = z0 + zseg < D ? z0 + zseg : D;
     const int wi = blockIdx.x / nseg / T::BLOCKS, b = (blockIdx.x / nseg) % T::BLOCKS;
     const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;
-    const DevWorld &w = a.worlds[a.ids[wi]];
+    // a copy, loaded once before any store (through a reference its fields would be
+    // reloaded after every store to the pool, which the compiler must assume aliases it)
+    const DevWorld w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C;
     const int kind = w.sobel_border, faces = (w.quirks & LHQ_FACE_HALO) != 0;
     // the non-blur central-difference copy (reintegration.cpp:179-186) leaves U_S zero
@@ -652,7 +657,9 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
     const int z0 = (blockIdx.x % nseg) * zseg, z1 = z0 + zseg < D ? z0 + zseg : D;
     const int wi = blockIdx.x / nseg / T::BLOCKS, b = (blockIdx.x / nseg) % T::BLOCKS;
     const int tx0 = (b % T::XT) * T::TX, ty0 = (b / T::XT) * T::TY;
-    const DevWorld &w = a.worlds[a.ids[wi]];
+    // a copy, loaded once before any store (through a reference its fields would be
+    // reloaded after every store to the pool, which the compiler must assume aliases it)
+    const DevWorld w = a.worlds[a.ids[wi]];
     if (!sweep_world(w)) return;
     const int tid = threadIdx.x, C = w.C;
     const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A_cur(a, w) : a.pool + w.Bk;
