@@ -676,7 +676,9 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
     // it is readable with z inside / outside the world (halo_src split per axis)
     unsigned off[T::ITEMS];
     bool ok_in[T::ITEMS], ok_out[T::ITEMS];
-    float posx[T::ITEMS], posy[T::ITEMS], shx[T::ITEMS], shy[T::ITEMS];
+    // destination centres toward which an item's weights are formed, slot + 1 - kk for
+    // kk = 0, 1, 2 (exact: small integers + 0.5)
+    float px[T::ITEMS][3], py[T::ITEMS][3], shx[T::ITEMS], shy[T::ITEMS];
 #pragma unroll
     for (int k = 0; k < T::ITEMS; k++) {
         const int item = tid + k * T::THREADS;
@@ -689,8 +691,11 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
         ok_in[k] = valid && (m == 0 || (tkind != 2 && !(faces && m >= 2)));
         ok_out[k] = valid && tkind != 2 && !(faces && m + 1 >= 2);
         off[k] = valid ? (unsigned)(c * 3 * S::N + xr + W * yr) : 0u;  // MU of channel c; mass at c*N
-        posx[k] = (float)x + 0.5f;
-        posy[k] = (float)y + 0.5f;
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) {
+            px[k][kk] = (float)(x + 1 - kk) + 0.5f;
+            py[k][kk] = (float)(y + 1 - kk) + 0.5f;
+        }
         shx[k] = !torus ? 0.0f : (x < 0 ? (float)W : (x >= W ? -(float)W : 0.0f));
         shy[k] = !torus ? 0.0f : (y < 0 ? (float)H : (y >= H ? -(float)H : 0.0f));
     }
@@ -720,7 +725,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
         if (zp < z1) fetch(zp + 1, nxt);
         __syncthreads();  // the previous plane's weights are no longer read
         const float shz = !torus ? 0.0f : (zp < 0 ? (float)D : (zp >= D ? -(float)D : 0.0f));
-        const float posz = (float)zp + 0.5f;
+        const float pz[3] = {(float)(zp + 1) + 0.5f, (float)zp + 0.5f, (float)(zp - 1) + 0.5f};
 #pragma unroll
         for (int k = 0; k < T::ITEMS; k++) {
             const int item = tid + k * T::THREADS;
@@ -728,18 +733,21 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
             const int c = item / T::SP, q = item % T::SP;
             float *Wc = sw_dyn + c * T::LDS_FLOATS_PER_CH;
             Wc[q] = cur[k][3];
-            const float p[3] = {posx[k], posy[k], posz}, sh[3] = {shx[k], shy[k], shz};
+            const float sh[3] = {shx[k], shy[k], shz};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const float mu = cur[k][d];
 #pragma unroll
                 for (int kk = 0; kk < 3; kk++) {
                     // destination coordinate slot + 1 - kk, centre (float)(dest) + 0.5
-                    const float pos = kk == 1 ? p[d] : (kk == 0 ? p[d] + 1.0f : p[d] - 1.0f);
+                    const float pos = d == 0 ? px[k][kk] : (d == 1 ? py[k][kk] : pz[kk]);
                     float dv = pos - mu;
                     dv = dv + sh[d];
                     dv = si - __builtin_fabsf(dv);
-                    Wc[(1 + d * 3 + kk) * T::SP + q] = __builtin_fminf(__builtin_fmaxf(dv, 0.0f), clip);
+                    // clamp(dv, 0, clip) in one instruction: for finite dv and clip > 0 the
+                    // median of (dv, 0, clip) is min(max(dv, 0), clip) (a zero's sign may
+                    // differ, which no sum of these products can observe)
+                    Wc[(1 + d * 3 + kk) * T::SP + q] = __builtin_amdgcn_fmed3f(dv, 0.0f, clip);
                 }
             }
         }
