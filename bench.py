@@ -49,11 +49,16 @@ def profiled_traffic(kernel, worlds):
     4096-world workload only; None when absent."""
     if worlds != 4096 or not os.path.exists(PROFILE_SUMMARY):
         return None
+    col = None
     for ln in open(PROFILE_SUMMARY):
         cols = [c.strip() for c in ln.strip().strip("|").split("|")]
-        if len(cols) > 7 and cols[0] == kernel:
+        if col is None:
+            if cols and cols[0] == "kernel":
+                col = next((j for j, c in enumerate(cols) if c.startswith("HBM GB")), None)
+            continue
+        if len(cols) > col and cols[0] == kernel:
             try:
-                return float(cols[7]) * 1e9
+                return float(cols[col]) * 1e9
             except ValueError:
                 return None
     return None
