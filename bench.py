@@ -5,14 +5,14 @@ periodic border, reference growth table), synthetic seeded worlds resident in HB
 next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
-                  [--no-mixed] [--no-precision] [--no-proposal]
+                  [--no-mixed] [--no-precision] [--no-proposal] [--no-checkpoint]
 
 A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
 per-world SRT or RT) and reports it under "mixed_batch"; a third steps one 128^3 world
 with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
 ("precision_128", with the matrix-core roofline of its dominant pass); a fourth times
 the proposal network ("proposal": scoring 10^6 sampled parameter sets in one call,
-training throughput).
+training throughput); "checkpoint" times save and restore of world checkpoint images.
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -185,6 +185,26 @@ def proposal_leg():
                          "flops_per_set": P.FLOPS_PER_SET}}
 
 
+def checkpoint_leg(eng, n):
+    """Checkpoint images of the first n worlds of a batch engine: save all, restore all
+    (host buffers in pageable memory; PCIe-bound), and a check that the restored state
+    reads back bit for bit."""
+    eng.synchronize()
+    t0 = time.perf_counter()
+    images = [eng.checkpoint(i) for i in range(n)]
+    t_save = time.perf_counter() - t0
+    before = eng.get(n - 1, "A")
+    t0 = time.perf_counter()
+    for i, img in enumerate(images):
+        eng.restore(i, img)
+    t_load = time.perf_counter() - t0
+    total = sum(b.nbytes for b in images)
+    return {"worlds": n, "image_bytes": images[0].nbytes, "save_GBs": total / t_save / 1e9,
+            "load_GBs": total / t_load / 1e9, "save_ms_per_world": t_save * 1e3 / n,
+            "load_ms_per_world": t_load * 1e3 / n,
+            "restored_equal": bool(np.array_equal(eng.get(n - 1, "A").view(np.uint32), before.view(np.uint32)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +220,8 @@ def main():
                     help="skip the proposal-network leg (scoring 10^6 sampled sets, training throughput)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-shape leg (24^3, 32^3, 36^3, 44x24x44, per-world SRT/RT)")
+    ap.add_argument("--no-checkpoint", action="store_true",
+                    help="skip timing checkpoint images (save and restore of 256 worlds)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -216,6 +238,7 @@ def main():
     value = args.worlds * args.steps / dt
     ms_per_step = dt * 1e3 / args.steps
     gpu_final = {seeds[i]: eng.get(i, "A") for i in range(len(seeds))}
+    ckpt = None if args.no_checkpoint else checkpoint_leg(eng, min(args.worlds, 256))
     eng.close()
 
     single_rate, sdt = float("nan"), float("nan")
@@ -237,6 +260,9 @@ def main():
                          "ms_per_step": sdt * 1e3 / max(args.steps, 100)},
         "batched_over_single": value / single_rate,
     }
+
+    if ckpt is not None:
+        line["checkpoint"] = ckpt
 
     if not args.no_mixed:
         mixed = WL.mixed(args.worlds, base_seed=1000)

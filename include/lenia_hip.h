@@ -160,6 +160,44 @@ lh_status lh_world_moments(void *engine, double *out, int n_worlds);
 lh_status lh_mc_case(void *engine, int world, int field, int channel, float iso,
                      int face_only, uint8_t *out);
 
+/* ---- checkpoints ----------------------------------------------------------------- */
+/* One world's complete state as a flat, versioned byte image: the intent of the
+ * reference's disabled save_config / load_config (config.cpp:75-261: the simulation
+ * globals, then per cell grid, GA, SUM and V, then channel flags and kernels), for one
+ * world of a batch.  Little-endian, in this order:
+ *   lh_checkpoint_header (64 bytes), lh_params, lh_kernel[K],
+ *   float32 A [C][N], GA [C][N], S [N], V [C][3][N]   (N = W*H*D, x fastest).
+ * The spectra, growth tables and per-step outputs (H, B, MU, U_SUM) are derived from
+ * these and are not stored. */
+#define LH_CHECKPOINT_VERSION 1u
+typedef struct {
+    char     magic[4];      /* "LHCK" */
+    uint32_t version;       /* LH_CHECKPOINT_VERSION */
+    uint32_t header_bytes;  /* sizeof(lh_checkpoint_header) = 64 */
+    uint32_t params_bytes;  /* sizeof(lh_params) */
+    uint32_t kernel_bytes;  /* sizeof(lh_kernel) */
+    int32_t  conv;          /* growth convolution (LH_CONV_*) */
+    lh_shape shape;
+    uint32_t reserved;      /* 0 */
+    uint64_t steps;         /* steps the world has taken (lh_world_steps) */
+    uint64_t payload_bytes; /* bytes of the float32 fields that follow the kernels */
+} lh_checkpoint_header;
+
+/* Size in bytes of world's checkpoint image. */
+lh_status lh_checkpoint_size(void *engine, int world, size_t *bytes);
+/* Writes world's image into buf (bytes >= lh_checkpoint_size); synchronous, after the
+ * steps enqueued so far. */
+lh_status lh_checkpoint_save(void *engine, int world, void *buf, size_t bytes);
+/* Restores world's state (A, GA, S, V, step counter) from an image.  The world must have
+ * the image's shape, parameters, kernels and convolution, bit for bit (LH_EINVAL
+ * otherwise, as for a wrong magic, version, size or truncated image); the spectra are
+ * re-derived before the next step, so stepping continues exactly as it would have
+ * from the saved state. */
+lh_status lh_checkpoint_load(void *engine, int world, const void *buf, size_t bytes);
+/* Steps world has taken since it was added (or since the step count of the image it
+ * was last loaded from). */
+lh_status lh_world_steps(void *engine, int world, uint64_t *steps);
+
 /* ---- introspection -------------------------------------------------------------- */
 int lh_world_count(void *engine);
 /* Number of distinct kernel names with recorded launches (LH_OPT_PROFILE). */

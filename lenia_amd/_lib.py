@@ -29,6 +29,7 @@ EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commi
            "lh_stream", "lh_world_mass", "lh_mc_case", "lh_world_count", "lh_profile_count",
            "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version",
            "lh_set_world_conv", "lh_world_moments",
+           "lh_checkpoint_size", "lh_checkpoint_save", "lh_checkpoint_load", "lh_world_steps",
            "lm_param_count", "lm_create", "lm_destroy", "lm_set_params", "lm_get_params", "lm_set_data",
            "lm_train", "lm_gradients", "lm_forward", "lm_score_sampled", "lm_sample_features")
 
@@ -60,6 +61,18 @@ class Kernel(C.Structure):
 
 class Shape(C.Structure):
     _fields_ = [("W", C.c_int), ("H", C.c_int), ("D", C.c_int), ("C", C.c_int), ("K", C.c_int)]
+
+
+CHECKPOINT_VERSION = 1
+
+
+class CheckpointHeader(C.Structure):
+    """lh_checkpoint_header (64 bytes)."""
+    _fields_ = [
+        ("magic", C.c_char * 4), ("version", C.c_uint32), ("header_bytes", C.c_uint32),
+        ("params_bytes", C.c_uint32), ("kernel_bytes", C.c_uint32), ("conv", C.c_int32),
+        ("shape", Shape), ("reserved", C.c_uint32), ("steps", C.c_uint64), ("payload_bytes", C.c_uint64),
+    ]
 
 
 class LeniaHipError(RuntimeError):
@@ -95,6 +108,10 @@ def load():
         "lh_stream": (vp, [vp]),
         "lh_world_mass": (i, [vp, C.POINTER(C.c_double), i]),
         "lh_world_moments": (i, [vp, C.POINTER(C.c_double), i]),
+        "lh_checkpoint_size": (i, [vp, i, C.POINTER(st)]),
+        "lh_checkpoint_save": (i, [vp, i, vp, st]),
+        "lh_checkpoint_load": (i, [vp, i, vp, st]),
+        "lh_world_steps": (i, [vp, i, C.POINTER(C.c_uint64)]),
         "lh_mc_case": (i, [vp, i, i, i, f, i, C.POINTER(C.c_uint8)]),
         "lh_world_count": (i, [vp]),
         "lh_profile_count": (i, [vp]),

@@ -124,6 +124,7 @@ struct HostWorld {
     DevWorld dev;
     int cls;
     int conv = LH_CONV_FFT;
+    uint64_t steps = 0;  // steps taken (restored from a checkpoint image)
 };
 
 struct ShapeClass {
@@ -765,6 +766,7 @@ lh_status lh_step(void *p, int nsteps) {
         HIPCHK(hipGetLastError());
         e->spectra_dirty = false;
     }
+    for (HostWorld &w : e->worlds) w.steps += (uint64_t)nsteps;
     int left = nsteps;
     if (e->opt_graph && !e->opt_profile) {
         while (left >= 2) {
@@ -807,6 +809,109 @@ lh_status lh_world_mass(void *p, double *out, int n) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, e->d_mass, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+// ---- checkpoints ---------------------------------------------------------------------
+// Image: lh_checkpoint_header, lh_params, lh_kernel[K], then A, GA, S, V as float32 (the
+// state fields of lh_set_field, in that order).
+static const int CKPT_FIELDS[4] = {LH_F_A, LH_F_GA, LH_F_S, LH_F_V};
+
+static size_t ckpt_payload(const HostWorld &hw) {
+    size_t n = 0;
+    for (int f : CKPT_FIELDS) n += field_count(hw, f) * sizeof(float);
+    return n;
+}
+
+static size_t ckpt_bytes(const HostWorld &hw) {
+    return sizeof(lh_checkpoint_header) + sizeof(lh_params) + hw.k.size() * sizeof(lh_kernel) + ckpt_payload(hw);
+}
+
+lh_status lh_checkpoint_size(void *p, int world, size_t *bytes) {
+    Engine *e = as_engine(p);
+    if (!e || !bytes) return fail(LH_EINVAL, "null argument");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    *bytes = ckpt_bytes(e->worlds[world]);
+    return LH_OK;
+}
+
+lh_status lh_checkpoint_save(void *p, int world, void *buf, size_t bytes) {
+    Engine *e = as_engine(p);
+    if (!e || !buf) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    const HostWorld &hw = e->worlds[world];
+    if (bytes < ckpt_bytes(hw)) return fail(LH_EINVAL, "buffer smaller than lh_checkpoint_size");
+    lh_checkpoint_header h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, "LHCK", 4);
+    h.version = LH_CHECKPOINT_VERSION;
+    h.header_bytes = sizeof(lh_checkpoint_header);
+    h.params_bytes = sizeof(lh_params);
+    h.kernel_bytes = sizeof(lh_kernel);
+    h.conv = hw.conv;
+    h.shape = hw.shape;
+    h.steps = hw.steps;
+    h.payload_bytes = ckpt_payload(hw);
+    char *out = static_cast<char *>(buf);
+    std::memcpy(out, &h, sizeof h);
+    out += sizeof h;
+    std::memcpy(out, &hw.p, sizeof(lh_params));
+    out += sizeof(lh_params);
+    std::memcpy(out, hw.k.data(), hw.k.size() * sizeof(lh_kernel));
+    out += hw.k.size() * sizeof(lh_kernel);
+    HIPCHK(hipSetDevice(e->device));
+    for (int f : CKPT_FIELDS) {
+        const size_t nb = field_count(hw, f) * sizeof(float);
+        HIPCHK(hipMemcpyAsync(out, e->pool + field_off(*e, hw, f), nb, hipMemcpyDeviceToHost, e->stream));
+        out += nb;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+lh_status lh_checkpoint_load(void *p, int world, const void *buf, size_t bytes) {
+    Engine *e = as_engine(p);
+    if (!e || !buf) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    HostWorld &hw = e->worlds[world];
+    if (bytes < sizeof(lh_checkpoint_header)) return fail(LH_EINVAL, "checkpoint image truncated");
+    lh_checkpoint_header h;
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, "LHCK", 4) != 0) return fail(LH_EINVAL, "not a checkpoint image (magic)");
+    if (h.version != LH_CHECKPOINT_VERSION) return fail(LH_EINVAL, "checkpoint version not supported");
+    if (h.header_bytes != sizeof(lh_checkpoint_header) || h.params_bytes != sizeof(lh_params) ||
+        h.kernel_bytes != sizeof(lh_kernel))
+        return fail(LH_EINVAL, "checkpoint written with a different layout");
+    if (bytes != ckpt_bytes(hw) || h.payload_bytes != ckpt_payload(hw))
+        return fail(LH_EINVAL, "checkpoint size does not match this world");
+    if (std::memcmp(&h.shape, &hw.shape, sizeof(lh_shape)) != 0 || h.conv != hw.conv)
+        return fail(LH_EINVAL, "checkpoint of a world with another shape or convolution");
+    const char *in = static_cast<const char *>(buf) + sizeof h;
+    if (std::memcmp(in, &hw.p, sizeof(lh_params)) != 0)
+        return fail(LH_EINVAL, "checkpoint of a world with other parameters");
+    in += sizeof(lh_params);
+    if (std::memcmp(in, hw.k.data(), hw.k.size() * sizeof(lh_kernel)) != 0)
+        return fail(LH_EINVAL, "checkpoint of a world with other kernels");
+    in += hw.k.size() * sizeof(lh_kernel);
+    HIPCHK(hipSetDevice(e->device));
+    for (int f : CKPT_FIELDS) {
+        const size_t nb = field_count(hw, f) * sizeof(float);
+        HIPCHK(hipMemcpyAsync(e->pool + field_off(*e, hw, f), in, nb, hipMemcpyHostToDevice, e->stream));
+        in += nb;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    hw.steps = h.steps;
+    e->spectra_dirty = true;
+    return LH_OK;
+}
+
+lh_status lh_world_steps(void *p, int world, uint64_t *steps) {
+    Engine *e = as_engine(p);
+    if (!e || !steps) return fail(LH_EINVAL, "null argument");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    *steps = e->worlds[world].steps;
     return LH_OK;
 }
 

@@ -56,8 +56,41 @@ class Engine:
         self.worlds.append((tuple(shape), C_, len(kernels)))
         return wid.value
 
+    def add_world_from_checkpoint(self, image: bytes) -> int:
+        """Adds a world with the shape, parameters, kernels and convolution recorded in a
+        checkpoint image (its state is loaded with restore() after commit)."""
+        ck = read_checkpoint(image)
+        wid = C.c_int()
+        L.check(self._L.lh_add_world(self._h, C.byref(ck["shape"]), C.byref(ck["params"]), ck["kernels"],
+                                     C.byref(wid)), "lh_add_world")
+        if ck["conv"] != L.CONV["fft"]:
+            L.check(self._L.lh_set_world_conv(self._h, wid.value, ck["conv"]), "lh_set_world_conv")
+        s = ck["shape"]
+        self.worlds.append(((s.W, s.H, s.D), s.C, s.K))
+        return wid.value
+
     def commit(self):
         L.check(self._L.lh_commit(self._h), "lh_commit")
+
+    def checkpoint(self, wid) -> np.ndarray:
+        """The world's checkpoint image (lh_checkpoint_save: header, parameters, kernels,
+        A, GA, S, V) as a uint8 array (bytes-like: write it to a file as is)."""
+        n = C.c_size_t()
+        L.check(self._L.lh_checkpoint_size(self._h, wid, C.byref(n)), "lh_checkpoint_size")
+        buf = np.empty(n.value, dtype=np.uint8)
+        L.check(self._L.lh_checkpoint_save(self._h, wid, buf.ctypes.data, n.value), "lh_checkpoint_save")
+        return buf
+
+    def restore(self, wid, image):
+        """Loads a checkpoint image (any bytes-like object) into a world with the same
+        shape, parameters, kernels and convolution (lh_checkpoint_load)."""
+        b = np.ascontiguousarray(np.frombuffer(image, dtype=np.uint8))
+        L.check(self._L.lh_checkpoint_load(self._h, wid, b.ctypes.data, b.size), "lh_checkpoint_load")
+
+    def steps(self, wid) -> int:
+        n = C.c_uint64()
+        L.check(self._L.lh_world_steps(self._h, wid, C.byref(n)), "lh_world_steps")
+        return n.value
 
     def _count(self, wid, field):
         (W, H, D), C_, _ = self.worlds[wid]
@@ -146,3 +179,33 @@ class Engine:
 
 def shape_supported(W, H, D) -> bool:
     return bool(L.load().lh_shape_supported(W, H, D))
+
+
+def read_checkpoint(image: bytes) -> dict:
+    """Parses a checkpoint image (layout of lh_checkpoint_header in lenia_hip.h) without
+    the engine: header fields, the lh_params / lh_kernel structures and the state fields
+    as float32 arrays (A [C][N], GA [C][N], S [N], V [C][3][N])."""
+    b = bytes(image)  # one copy; the arrays below are independent of the caller's buffer
+    hs = C.sizeof(L.CheckpointHeader)
+    if len(b) < hs:
+        raise ValueError("checkpoint image truncated")
+    h = L.CheckpointHeader.from_buffer_copy(b[:hs])
+    if h.magic != b"LHCK" or h.version != L.CHECKPOINT_VERSION or h.header_bytes != hs:
+        raise ValueError("not a version-1 checkpoint image")
+    if h.params_bytes != C.sizeof(L.Params) or h.kernel_bytes != C.sizeof(L.Kernel):
+        raise ValueError("checkpoint written with a different structure layout")
+    s = h.shape
+    N = s.W * s.H * s.D
+    sizes = {"A": s.C * N, "GA": s.C * N, "S": N, "V": 3 * s.C * N}
+    off = hs + h.params_bytes + s.K * h.kernel_bytes
+    if h.payload_bytes != 4 * sum(sizes.values()) or len(b) != off + h.payload_bytes:
+        raise ValueError("checkpoint size does not match its header")
+    params = L.Params.from_buffer_copy(b[hs:hs + h.params_bytes])
+    kernels = (L.Kernel * s.K).from_buffer_copy(b[hs + h.params_bytes:off])
+    fields = {}
+    for name, n in sizes.items():
+        fields[name] = np.frombuffer(b, dtype="<f4", count=n, offset=off).copy()
+        off += 4 * n
+    shape = L.Shape(s.W, s.H, s.D, s.C, s.K)
+    return dict(version=h.version, steps=h.steps, conv=h.conv, shape=shape, params=params, kernels=kernels,
+                fields=fields)
