@@ -5,14 +5,15 @@ periodic border, reference growth table), synthetic seeded worlds resident in HB
 next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
-                  [--no-mixed] [--no-precision] [--no-proposal] [--no-checkpoint]
+                  [--no-mixed] [--no-precision] [--no-proposal] [--no-checkpoint] [--no-sampler]
 
 A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
 per-world SRT or RT) and reports it under "mixed_batch"; a third steps one 128^3 world
 with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
 ("precision_128", with the matrix-core roofline of its dominant pass); a fourth times
 the proposal network ("proposal": scoring 10^6 sampled parameter sets in one call,
-training throughput); "checkpoint" times save and restore of world checkpoint images.
+training throughput); "checkpoint" times save and restore of world checkpoint images;
+"random_worlds" times the generate_random-range sampler and the Gaussian seeding.
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -205,6 +206,26 @@ def checkpoint_leg(eng, n):
             "restored_equal": bool(np.array_equal(eng.get(n - 1, "A").view(np.uint32), before.view(np.uint32)))}
 
 
+def sampler_leg(eng, n_worlds):
+    """generate_random-range parameter sets sampled on the GPU (10^6 in one call, into
+    host lh_kernel buffers: 80 MB) and every world of the batch engine seeded with its set's Gaussian blobs
+    (A written, GA, S, V cleared, on the GPU)."""
+    from lenia_amd.engine import sample_random_raw
+    sample_random_raw(5, 0, 1000, 3)
+    t0 = time.perf_counter()
+    _, counts = sample_random_raw(5, 0, 1_000_000, 3)
+    t_sample = time.perf_counter() - t0
+    eng.seed_random(0, n_worlds, 5, 0)
+    t0 = time.perf_counter()
+    eng.seed_random(0, n_worlds, 5, 0)
+    t_seed = time.perf_counter() - t0
+    written = n_worlds * 32 ** 3 * (3 + 3 + 1 + 9) * 4
+    return {"workload": "generate_random ranges, 3 channels; seeding 32^3 x 3 worlds",
+            "sample_sets_per_s": 1e6 / t_sample, "sample_ms_1e6": t_sample * 1e3,
+            "mean_kernels_per_set": float(counts.mean()),
+            "seed_worlds_per_s": n_worlds / t_seed, "seed_GBs_written": written / t_seed / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +243,8 @@ def main():
                     help="skip the mixed-shape leg (24^3, 32^3, 36^3, 44x24x44, per-world SRT/RT)")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip timing checkpoint images (save and restore of 256 worlds)")
+    ap.add_argument("--no-sampler", action="store_true",
+                    help="skip timing the random-world sampler (10^6 sets) and seeding (every world)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -239,6 +262,7 @@ def main():
     ms_per_step = dt * 1e3 / args.steps
     gpu_final = {seeds[i]: eng.get(i, "A") for i in range(len(seeds))}
     ckpt = None if args.no_checkpoint else checkpoint_leg(eng, min(args.worlds, 256))
+    sampler = None if args.no_sampler else sampler_leg(eng, args.worlds)
     eng.close()
 
     single_rate, sdt = float("nan"), float("nan")
@@ -263,6 +287,8 @@ def main():
 
     if ckpt is not None:
         line["checkpoint"] = ckpt
+    if sampler is not None:
+        line["random_worlds"] = sampler
 
     if not args.no_mixed:
         mixed = WL.mixed(args.worlds, base_seed=1000)

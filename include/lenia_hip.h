@@ -198,6 +198,23 @@ lh_status lh_checkpoint_load(void *engine, int world, const void *buf, size_t by
  * was last loaded from). */
 lh_status lh_world_steps(void *engine, int world, uint64_t *steps);
 
+/* ---- random worlds (generate_random lenia.cpp:370-429, seed_grid_gaussian
+ * lenia.cpp:305-334) -------------------------------------------------------------- */
+/* Samples parameter sets first .. first+n-1 on the GPU with generate_random's ranges:
+ * neighbourhood 1..8 (every kernel's radius), 1..10 kernels, 1..2 rings with peaks
+ * k/100, m = (100 + k)/1000 (k < 500), s = m/10 + k/1000 (k < 200), h = 1, r_a = 1,
+ * c0 = c1 = 0 for kernel 0 or one channel, else uniform over the channels.  Draws are
+ * splitmix64 of (seed, set, slot) in place of rand(), so a set is reproducible from its
+ * index.  kernels[n][LH_MAX_K] (entries past the count zeroed), kernel_count[n]; host
+ * buffers, synchronous. */
+lh_status lh_sample_random(int device, uint64_t seed, int64_t first, int n, int channels,
+                           lh_kernel *kernels, int *kernel_count);
+/* Seeds worlds first_world .. first_world+n_worlds-1 on the GPU: world j gets the five
+ * Gaussian blobs of set first_set + j (centres drawn like the kernels, width its
+ * neighbourhood, every channel, plain distance) in A, and zero GA, S, V and step count,
+ * as generate_random does after sampling.  Synchronous. */
+lh_status lh_seed_random(void *engine, int first_world, int n_worlds, uint64_t seed, int64_t first_set);
+
 /* ---- introspection -------------------------------------------------------------- */
 int lh_world_count(void *engine);
 /* Number of distinct kernel names with recorded launches (LH_OPT_PROFILE). */

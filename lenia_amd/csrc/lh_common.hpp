@@ -82,6 +82,15 @@ struct ShapeOps {
 
 const ShapeOps *lh_shape_ops(int W, int H, int D);
 
+// One world to seed (lh_seed_random): float offsets of its current A, GA, S, V in the
+// pool, and its extents.
+struct SeedJob {
+    int64_t A, GA, S, V;
+    int W, H, D, C;
+};
+void lh_launch_seed_random(float *pool, const SeedJob *d_jobs, int n, int max_cells, uint64_t seed,
+                           long long first_set, hipStream_t st);
+
 // Growth convolution as DFT-as-GEMM on matrix cores (cubic worlds of side N); arg of
 // fx/fy/fz: channels per world, of iz/iy: kernels per world.
 struct GemmOps {

@@ -907,6 +907,44 @@ lh_status lh_checkpoint_load(void *p, int world, const void *buf, size_t bytes) 
     return LH_OK;
 }
 
+lh_status lh_seed_random(void *p, int first_world, int n_worlds, uint64_t seed, int64_t first_set) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (n_worlds < 0 || first_world < 0 || first_world + n_worlds > (int)e->worlds.size())
+        return fail(LH_EINVAL, "world range outside the engine");
+    if (n_worlds == 0) return LH_OK;
+    std::vector<SeedJob> jobs(n_worlds);
+    int max_cells = 0;
+    for (int j = 0; j < n_worlds; j++) {
+        const HostWorld &hw = e->worlds[first_world + j];
+        SeedJob &jb = jobs[j];
+        jb.A = field_off(*e, hw, LH_F_A);
+        jb.GA = field_off(*e, hw, LH_F_GA);
+        jb.S = field_off(*e, hw, LH_F_S);
+        jb.V = field_off(*e, hw, LH_F_V);
+        jb.W = hw.shape.W;
+        jb.H = hw.shape.H;
+        jb.D = hw.shape.D;
+        jb.C = hw.shape.C;
+        max_cells = std::max(max_cells, jb.W * jb.H * jb.D);
+    }
+    HIPCHK(hipSetDevice(e->device));
+    SeedJob *dj = nullptr;
+    HIPCHK(hipMalloc(&dj, jobs.size() * sizeof(SeedJob)));
+    hipError_t r = hipMemcpyAsync(dj, jobs.data(), jobs.size() * sizeof(SeedJob), hipMemcpyHostToDevice, e->stream);
+    if (r == hipSuccess) {
+        lh_launch_seed_random(e->pool, dj, n_worlds, max_cells, seed, (long long)first_set, e->stream);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    (void)hipFree(dj);
+    if (r != hipSuccess) return fail(LH_EHIP, std::string("lh_seed_random: ") + hipGetErrorString(r));
+    for (int j = 0; j < n_worlds; j++) e->worlds[first_world + j].steps = 0;
+    e->spectra_dirty = true;
+    return LH_OK;
+}
+
 lh_status lh_world_steps(void *p, int world, uint64_t *steps) {
     Engine *e = as_engine(p);
     if (!e || !steps) return fail(LH_EINVAL, "null argument");

@@ -87,6 +87,12 @@ class Engine:
         b = np.ascontiguousarray(np.frombuffer(image, dtype=np.uint8))
         L.check(self._L.lh_checkpoint_load(self._h, wid, b.ctypes.data, b.size), "lh_checkpoint_load")
 
+    def seed_random(self, first_world, n_worlds, seed, first_set=0):
+        """Seeds worlds first_world.. with the Gaussian blobs of sampled sets first_set..
+        (lh_seed_random: A from five blobs, GA, S, V and step count cleared)."""
+        L.check(self._L.lh_seed_random(self._h, int(first_world), int(n_worlds), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                       int(first_set)), "lh_seed_random")
+
     def steps(self, wid) -> int:
         n = C.c_uint64()
         L.check(self._L.lh_world_steps(self._h, wid, C.byref(n)), "lh_world_steps")
@@ -179,6 +185,32 @@ class Engine:
 
 def shape_supported(W, H, D) -> bool:
     return bool(L.load().lh_shape_supported(W, H, D))
+
+
+def sample_random_raw(seed, first, n, channels, device=0):
+    """lh_sample_random into fresh host buffers: (lh_kernel array [n * LH_MAX_K], kernel
+    counts [n])."""
+    lib = L.load()
+    ks = (L.Kernel * max(1, n * L.MAX_K))()
+    cnt = np.zeros(max(1, n), dtype=np.int32)
+    L.check(lib.lh_sample_random(int(device), int(seed) & 0xFFFFFFFFFFFFFFFF, int(first), int(n), int(channels),
+                                 ks, cnt.ctypes.data_as(C.POINTER(C.c_int))), "lh_sample_random")
+    return ks, cnt[:n]
+
+
+def sample_random(seed, first, n, channels, device=0) -> list:
+    """Parameter sets first .. first+n-1 sampled on the GPU with generate_random's ranges
+    (lh_sample_random): one list of kernel dicts (add_world's format) per set."""
+    ks, cnt = sample_random_raw(seed, first, n, channels, device)
+    out = []
+    for j in range(n):
+        sets = []
+        for i in range(cnt[j]):
+            k = ks[j * L.MAX_K + i]
+            sets.append(dict(c0=k.c0, c1=k.c1, radius=k.radius, beta_size=k.beta_size,
+                             b=[k.b[e] for e in range(k.beta_size)], r_a=k.r_a, m=k.m, s=k.s, h=k.h))
+        out.append(sets)
+    return out
 
 
 def read_checkpoint(image: bytes) -> dict:
