@@ -13,7 +13,8 @@ with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
 ("precision_128", with the matrix-core roofline of its dominant pass); a fourth times
 the proposal network ("proposal": scoring 10^6 sampled parameter sets in one call,
 training throughput); "checkpoint" times save and restore of world checkpoint images;
-"random_worlds" times the generate_random-range sampler and the Gaussian seeding.
+"random_worlds" times the generate_random-range sampler and the Gaussian seeding;
+"snapshot_readback" reads a snapshot back while later steps run.
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -206,6 +207,24 @@ def checkpoint_leg(eng, n):
             "restored_equal": bool(np.array_equal(eng.get(n - 1, "A").view(np.uint32), before.view(np.uint32)))}
 
 
+def readback_leg(eng, n, steps, step_s):
+    """A of n worlds read back from a snapshot while `steps` further steps run: the host
+    reads finish without waiting for those steps, which keep their speed."""
+    eng.synchronize()
+    eng.snapshot(0, 0, n, ("A",))
+    t0 = time.perf_counter()
+    eng.step(steps)
+    for i in range(n):
+        eng.snapshot_read(0, i, "A")
+    t_read = time.perf_counter() - t0
+    eng.synchronize()
+    t_all = time.perf_counter() - t0
+    nbytes = n * 3 * 32 ** 3 * 4
+    return {"worlds": n, "bytes": nbytes, "read_GBs": nbytes / t_read / 1e9, "read_ms": t_read * 1e3,
+            "steps_ms_with_reads": t_all * 1e3, "steps_ms_alone": step_s * steps * 1e3,
+            "reads_done_before_steps": t_read < t_all}
+
+
 def sampler_leg(eng, n_worlds):
     """generate_random-range parameter sets sampled on the GPU (10^6 in one call, into
     host lh_kernel buffers: 80 MB) and every world of the batch engine seeded with its set's Gaussian blobs
@@ -263,6 +282,7 @@ def main():
     gpu_final = {seeds[i]: eng.get(i, "A") for i in range(len(seeds))}
     ckpt = None if args.no_checkpoint else checkpoint_leg(eng, min(args.worlds, 256))
     sampler = None if args.no_sampler else sampler_leg(eng, args.worlds)
+    readback = None if args.no_checkpoint else readback_leg(eng, min(args.worlds, 256), args.steps, dt / args.steps)
     eng.close()
 
     single_rate, sdt = float("nan"), float("nan")
@@ -289,6 +309,8 @@ def main():
         line["checkpoint"] = ckpt
     if sampler is not None:
         line["random_worlds"] = sampler
+    if readback is not None:
+        line["snapshot_readback"] = readback
 
     if not args.no_mixed:
         mixed = WL.mixed(args.worlds, base_seed=1000)

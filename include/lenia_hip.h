@@ -215,6 +215,23 @@ lh_status lh_sample_random(int device, uint64_t seed, int64_t first, int n, int 
  * as generate_random does after sampling.  Synchronous. */
 lh_status lh_seed_random(void *engine, int first_world, int n_worlds, uint64_t seed, int64_t first_set);
 
+/* ---- asynchronous read-backs ------------------------------------------------------ */
+/* The render side of the reference copies GRID -> RBB (and reads U_SUM_RB / H_RB) on its
+ * main thread while the simulation threads may be transporting (check_and_dispatch_threads
+ * main.cpp:212-244; the phase protocol render.cpp:357-604).  Here a snapshot is a device
+ * copy of selected fields, enqueued on the engine stream after every step enqueued so far
+ * and before any later one, so it is always one whole step's state; two slots let a
+ * consumer read one while the next is taken.  No reference counterpart as an API. */
+enum { LH_SNAP_A = 1, LH_SNAP_H = 2, LH_SNAP_USUM = 4 };
+/* Enqueues a snapshot of fields (mask of LH_SNAP_*) of worlds first_world ..
+ * first_world+n_worlds-1 into slot 0 or 1 and returns (U_SUM needs LH_OPT_KEEP_USUM). */
+lh_status lh_snapshot(void *engine, int slot, int first_world, int n_worlds, int fields);
+/* ready = 1 once the slot's copy has completed, 0 while it waits behind earlier steps. */
+lh_status lh_snapshot_ready(void *engine, int slot, int *ready);
+/* Copies one field (LH_F_A, LH_F_H or LH_F_USUM) of one world of the slot to host memory;
+ * waits for that snapshot only, not for steps enqueued after it. */
+lh_status lh_snapshot_read(void *engine, int slot, int world, int field, float *dst, size_t count);
+
 /* ---- introspection -------------------------------------------------------------- */
 int lh_world_count(void *engine);
 /* Number of distinct kernel names with recorded launches (LH_OPT_PROFILE). */

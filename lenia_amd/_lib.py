@@ -30,7 +30,7 @@ EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commi
            "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version",
            "lh_set_world_conv", "lh_world_moments",
            "lh_checkpoint_size", "lh_checkpoint_save", "lh_checkpoint_load", "lh_world_steps",
-           "lh_sample_random", "lh_seed_random",
+           "lh_sample_random", "lh_seed_random", "lh_snapshot", "lh_snapshot_ready", "lh_snapshot_read",
            "lm_param_count", "lm_create", "lm_destroy", "lm_set_params", "lm_get_params", "lm_set_data",
            "lm_train", "lm_gradients", "lm_forward", "lm_score_sampled", "lm_sample_features")
 
@@ -115,6 +115,9 @@ def load():
         "lh_world_steps": (i, [vp, i, C.POINTER(C.c_uint64)]),
         "lh_sample_random": (i, [i, C.c_uint64, C.c_int64, i, i, C.POINTER(Kernel), C.POINTER(i)]),
         "lh_seed_random": (i, [vp, i, i, C.c_uint64, C.c_int64]),
+        "lh_snapshot": (i, [vp, i, i, i, i]),
+        "lh_snapshot_ready": (i, [vp, i, C.POINTER(i)]),
+        "lh_snapshot_read": (i, [vp, i, i, i, C.POINTER(f), st]),
         "lh_mc_case": (i, [vp, i, i, i, f, i, C.POINTER(C.c_uint8)]),
         "lh_world_count": (i, [vp]),
         "lh_profile_count": (i, [vp]),

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <string>
@@ -150,9 +151,22 @@ struct Prof {
     double ms = 0.0;
 };
 
+// One read-back slot (lh_snapshot): device copies of selected fields of a range of
+// worlds, taken in stream order between steps.
+struct Snapshot {
+    float *d = nullptr;
+    size_t cap = 0;  // floats
+    hipEvent_t done = nullptr;
+    bool taken = false;
+    int first = 0, n = 0, fields = 0;
+    std::vector<size_t> off;  // [world - first][field bit] float offset in d
+};
+
 struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t rb_stream = nullptr;  // read-backs of snapshots (created on first use)
+    Snapshot snap[2];
     std::vector<HostWorld> worlds;
     std::vector<ShapeClass> classes;
     bool committed = false;
@@ -543,6 +557,11 @@ lh_status lh_destroy(void *p) {
     hipFree(e->d_mass);
     hipFree(e->d_mom);
     hipFree(e->d_mc);
+    for (Snapshot &sn : e->snap) {
+        hipFree(sn.d);
+        if (sn.done) hipEventDestroy(sn.done);
+    }
+    if (e->rb_stream) hipStreamDestroy(e->rb_stream);
     hipStreamDestroy(e->stream);
     delete e;
     return LH_OK;
@@ -942,6 +961,92 @@ lh_status lh_seed_random(void *p, int first_world, int n_worlds, uint64_t seed, 
     if (r != hipSuccess) return fail(LH_EHIP, std::string("lh_seed_random: ") + hipGetErrorString(r));
     for (int j = 0; j < n_worlds; j++) e->worlds[first_world + j].steps = 0;
     e->spectra_dirty = true;
+    return LH_OK;
+}
+
+// ---- snapshots ------------------------------------------------------------------------
+static const int SNAP_FIELD[3] = {LH_F_A, LH_F_H, LH_F_USUM};  // bit b of the mask
+
+lh_status lh_snapshot(void *p, int slot, int first_world, int n_worlds, int fields) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (slot < 0 || slot > 1) return fail(LH_EINVAL, "slot must be 0 or 1");
+    if (n_worlds < 1 || first_world < 0 || first_world + n_worlds > (int)e->worlds.size())
+        return fail(LH_EINVAL, "world range outside the engine");
+    if (fields <= 0 || (fields & ~(LH_SNAP_A | LH_SNAP_H | LH_SNAP_USUM)))
+        return fail(LH_EINVAL, "fields: a non-empty mask of LH_SNAP_A, LH_SNAP_H, LH_SNAP_USUM");
+    if ((fields & LH_SNAP_USUM) && !e->opt_keep_usum) return fail(LH_ESTATE, "U_SUM needs LH_OPT_KEEP_USUM");
+    HIPCHK(hipSetDevice(e->device));
+    Snapshot &sn = e->snap[slot];
+    std::vector<size_t> off((size_t)n_worlds * 3, SIZE_MAX);
+    size_t total = 0;
+    for (int j = 0; j < n_worlds; j++)
+        for (int b = 0; b < 3; b++)
+            if (fields & (1 << b)) {
+                off[(size_t)j * 3 + b] = total;
+                total += field_count(e->worlds[first_world + j], SNAP_FIELD[b]);
+            }
+    if (!sn.done) HIPCHK(hipEventCreateWithFlags(&sn.done, hipEventDisableTiming));
+    if (total > sn.cap) {
+        if (sn.d) {
+            HIPCHK(hipEventSynchronize(sn.done));  // copies into the old buffer have landed
+            HIPCHK(hipFree(sn.d));
+            sn.d = nullptr;
+            sn.cap = 0;
+        }
+        HIPCHK(hipMalloc(&sn.d, total * sizeof(float)));
+        sn.cap = total;
+    }
+    for (int j = 0; j < n_worlds; j++) {
+        const HostWorld &hw = e->worlds[first_world + j];
+        for (int b = 0; b < 3; b++)
+            if (fields & (1 << b))
+                HIPCHK(hipMemcpyAsync(sn.d + off[(size_t)j * 3 + b], e->pool + field_off(*e, hw, SNAP_FIELD[b]),
+                                      field_count(hw, SNAP_FIELD[b]) * sizeof(float), hipMemcpyDeviceToDevice,
+                                      e->stream));
+    }
+    HIPCHK(hipEventRecord(sn.done, e->stream));
+    sn.off.swap(off);
+    sn.first = first_world;
+    sn.n = n_worlds;
+    sn.fields = fields;
+    sn.taken = true;
+    return LH_OK;
+}
+
+lh_status lh_snapshot_ready(void *p, int slot, int *ready) {
+    Engine *e = as_engine(p);
+    if (!e || !ready) return fail(LH_EINVAL, "null argument");
+    if (slot < 0 || slot > 1) return fail(LH_EINVAL, "slot must be 0 or 1");
+    const Snapshot &sn = e->snap[slot];
+    if (!sn.taken) return fail(LH_ESTATE, "no snapshot in this slot");
+    HIPCHK(hipSetDevice(e->device));
+    const hipError_t q = hipEventQuery(sn.done);
+    if (q != hipSuccess && q != hipErrorNotReady) return fail(LH_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+    *ready = q == hipSuccess ? 1 : 0;
+    return LH_OK;
+}
+
+lh_status lh_snapshot_read(void *p, int slot, int world, int field, float *dst, size_t count) {
+    Engine *e = as_engine(p);
+    if (!e || !dst) return fail(LH_EINVAL, "null argument");
+    if (slot < 0 || slot > 1) return fail(LH_EINVAL, "slot must be 0 or 1");
+    const Snapshot &sn = e->snap[slot];
+    if (!sn.taken) return fail(LH_ESTATE, "no snapshot in this slot");
+    if (world < sn.first || world >= sn.first + sn.n) return fail(LH_EINVAL, "world not in this snapshot");
+    int b = -1;
+    for (int q = 0; q < 3; q++)
+        if (SNAP_FIELD[q] == field) b = q;
+    if (b < 0 || !(sn.fields & (1 << b))) return fail(LH_EINVAL, "field not in this snapshot");
+    const HostWorld &hw = e->worlds[world];
+    if (count != field_count(hw, field)) return fail(LH_EINVAL, "count does not match the field size");
+    HIPCHK(hipSetDevice(e->device));
+    if (!e->rb_stream) HIPCHK(hipStreamCreateWithFlags(&e->rb_stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamWaitEvent(e->rb_stream, sn.done, 0));
+    HIPCHK(hipMemcpyAsync(dst, sn.d + sn.off[(size_t)(world - sn.first) * 3 + b], count * sizeof(float),
+                          hipMemcpyDeviceToHost, e->rb_stream));
+    HIPCHK(hipStreamSynchronize(e->rb_stream));
     return LH_OK;
 }
 

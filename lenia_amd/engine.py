@@ -93,6 +93,27 @@ class Engine:
         L.check(self._L.lh_seed_random(self._h, int(first_world), int(n_worlds), int(seed) & 0xFFFFFFFFFFFFFFFF,
                                        int(first_set)), "lh_seed_random")
 
+    SNAP = {"A": 1, "H": 2, "USUM": 4}
+
+    def snapshot(self, slot, first_world=0, n_worlds=None, fields=("A",)):
+        """Enqueues a snapshot of fields of worlds first_world.. into slot 0 or 1, ordered
+        after the steps enqueued so far (lh_snapshot); returns at once."""
+        n = len(self.worlds) - first_world if n_worlds is None else n_worlds
+        mask = sum(self.SNAP[f] for f in set(fields))
+        L.check(self._L.lh_snapshot(self._h, int(slot), int(first_world), int(n), mask), "lh_snapshot")
+
+    def snapshot_ready(self, slot) -> bool:
+        r = C.c_int()
+        L.check(self._L.lh_snapshot_ready(self._h, int(slot), C.byref(r)), "lh_snapshot_ready")
+        return bool(r.value)
+
+    def snapshot_read(self, slot, wid, field="A") -> np.ndarray:
+        """One field of one world of a snapshot; waits for that snapshot only."""
+        a = np.empty(self._count(wid, field), dtype=np.float32)
+        L.check(self._L.lh_snapshot_read(self._h, int(slot), int(wid), self.FIELDS[field],
+                                         a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "lh_snapshot_read")
+        return a
+
     def steps(self, wid) -> int:
         n = C.c_uint64()
         L.check(self._L.lh_world_steps(self._h, wid, C.byref(n)), "lh_world_steps")
