@@ -126,6 +126,16 @@ struct TileCfg {
     __device__ static __forceinline__ int slot(int tx, int ty, int tz) { return tx + SX * (ty + SY * tz); }
 };
 
+// Element idx of a world array (idx < 2^30): a 32-bit byte offset from the array's
+// base, so the address is base + zext(offset) -- one 32-bit add with the base in scalar
+// registers, where a 64-bit index costs a 64-bit add per access.
+__device__ __forceinline__ float &el(float *base, unsigned idx) {
+    return *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (idx << 2));
+}
+__device__ __forceinline__ const float &el(const float *base, unsigned idx) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (idx << 2));
+}
+
 // The A value alpha reads per channel (PREALPHA index quirk applied; 0 when the quirk's
 // index falls outside the channels), loaded by the sweep one plane ahead of its use.
 template <int CM>
@@ -146,7 +156,7 @@ __device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w
         int cs = c, is = (int)i;
         bool ok = c < C;
         if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = ok && cs < C && is < S::N; }
-        in.aal[c] = ok ? A[cs * S::N + is] : 0.0f;
+        in.aal[c] = ok ? el(A, (unsigned)(cs * S::N + is)) : 0.0f;
     }
 }
 
@@ -184,22 +194,23 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
             // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
             const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
             const int gc = r / S::N;
-            const float ga = (r < LHD_MAX_C * S::N && gc < C) ? a.pool[w.GA + r] : 0.0f;
-            a.pool[w.AL + c * S::N + i] = __builtin_fabsf(A[c * S::N + i] - ga);
+            const float ga = (r < LHD_MAX_C * S::N && gc < C) ? el(a.pool + w.GA, (unsigned)r) : 0.0f;
+            el(a.pool + w.AL, (unsigned)(c * S::N + i)) = __builtin_fabsf(el(A, (unsigned)(c * S::N + i)) - ga);
         }
         const float al1 = 1.0f - al;
         float F[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) F[d] = nu[d] * al1 - na[d] * al;
-        float *V = a.pool + w.V + (long long)c * 3 * S::N;
+        float *V = a.pool + w.V;
+        const unsigned iu = (unsigned)i;
         float v[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) v[d] = w.v_live ? V[(long long)d * S::N + i] : 0.0f;
+        for (int d = 0; d < 3; d++) v[d] = w.v_live ? el(V, (unsigned)((3 * c + d) * S::N) + iu) : 0.0f;
         if (w.mu_vel) {
 #pragma unroll
             for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
         }
-        float *MU = a.pool + w.MU + (long long)c * 3 * S::N;
+        float *MU = a.pool + w.MU;
         if (w.mode == 2) {
             float vn[3];
 #pragma unroll
@@ -207,7 +218,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const float src = w.adv_vel ? vn[d] : F[d];
-                MU[(long long)d * S::N + i] = fmin_ref(fmax_ref(src * dt, nma), ma);
+                el(MU, (unsigned)((3 * c + d) * S::N) + iu) = fmin_ref(fmax_ref(src * dt, nma), ma);
             }
         } else {
 #pragma unroll
@@ -217,7 +228,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
                 m = fmin_ref(m, ma);
                 m = pos[d] + m;
                 if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
-                MU[(long long)d * S::N + i] = m;
+                el(MU, (unsigned)((3 * c + d) * S::N) + iu) = m;
             }
         }
         if (!w.mu_vel) {
@@ -226,12 +237,12 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
         }
         if (w.v_live) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) V[(long long)d * S::N + i] = v[d];
+            for (int d = 0; d < 3; d++) el(V, (unsigned)((3 * c + d) * S::N) + iu) = v[d];
         }
         if (w.need_f) {
-            float *Fo = a.pool + w.Fv + (long long)c * 3 * S::N;
+            float *Fo = a.pool + w.Fv;
 #pragma unroll
-            for (int d = 0; d < 3; d++) Fo[(long long)d * S::N + i] = F[d];
+            for (int d = 0; d < 3; d++) el(Fo, (unsigned)((3 * c + d) * S::N) + iu) = F[d];
         }
     }
 }
@@ -599,8 +610,8 @@ __device__ __forceinline__ void trm_load(const PhaseArgs &a, const DevWorld &w, 
     using S = Shape<W, H, D>;
     const float *GA = a.pool + w.GA;
 #pragma unroll
-    for (int c = 0; c < CM; c++) in.ga[c] = c < w.C ? GA[(long long)c * S::N + i] : 0.0f;
-    in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : a.pool[w.S + i];
+    for (int c = 0; c < CM; c++) in.ga[c] = c < w.C ? el(GA, (unsigned)(c * S::N + i)) : 0.0f;
+    in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : el(a.pool + w.S, (unsigned)i);
 }
 
 // C <= CM for every world this is used on.
@@ -616,21 +627,21 @@ __device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w,
 #pragma unroll
     for (int c = 0; c < CM; c++) {
         if (c >= C) break;
-        const long long g = (long long)c * S::N + i;
+        const unsigned g = (unsigned)(c * S::N + i);
         float t = tt[c];
         if (w.growth_type == 2) {
             const bool soft = (w.soft_mask >> c) & 1;
-            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) a.pool[w.Bk + g] = soft_growth(Acur[g], Hg[g], w.dt);
-            else if (soft) t = soft_growth(t, Hg[g], w.dt);
-            else t = clamp01 ? clamp_ref(t + Hg[g] * w.dt, 0.0f, 1.0f) : t + Hg[g] * w.dt;
+            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) el(a.pool + w.Bk, g) = soft_growth(el(Acur, g), el(Hg, g), w.dt);
+            else if (soft) t = soft_growth(t, el(Hg, g), w.dt);
+            else t = clamp01 ? clamp_ref(t + el(Hg, g) * w.dt, 0.0f, 1.0f) : t + el(Hg, g) * w.dt;
         } else if (clamp01) {
             t = clamp_ref(t, 0.0f, 1.0f);
         }
         ttsum = ttsum + t;
-        GA[g] = w.gtam * in.ga[c] + t * w.gtas;
-        An[g] = t;
+        el(GA, g) = w.gtam * in.ga[c] + t * w.gtas;
+        el(An, g) = t;
     }
-    Sg[i] = w.stam * in.s + ttsum * w.stas;
+    el(Sg, (unsigned)i) = w.stam * in.s + ttsum * w.stas;
 }
 
 // ------------------------------------------------------------------------------------
@@ -709,8 +720,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
             const unsigned o = off[k] + zoff;
             const int c = (tid + k * T::THREADS) / T::SP;
 #pragma unroll
-            for (int d = 0; d < 3; d++) buf[k][d] = ok ? MU[o + d * S::N] : 0.0f;
-            buf[k][3] = ok ? ARR[o - 2 * c * S::N] : 0.0f;
+            for (int d = 0; d < 3; d++) buf[k][d] = ok ? el(MU, o + d * S::N) : 0.0f;
+            buf[k][3] = ok ? el(ARR, o - 2 * c * S::N) : 0.0f;
         }
     };
     float accP[CM], accC[CM], accN[CM];
