@@ -167,123 +167,148 @@ __device__ __forceinline__ void gemm_real(const unsigned short *shi, const unsig
                 c[rb][lb][i] = (SPLIT ? c[rb][lb][i] + x[rb][lb][i] * MT<T>::UNLO : c[rb][lb][i]) * scale;
 }
 
-// Complex transform along y (AXIS 1) or z (AXIS 2) of 32 lines of a half spectrum.
+// Complex transform along y (AXIS 1) or z (AXIS 2) of 32-line tiles of a half spectrum.
 // IO 0: forward, in place on channel f of Fsp.  IO 1: inverse of (Fsp[c0] . K^_f) into
 // Gsp[f].  IO 2: inverse, in place on kernel f of Gsp.  arg = fields per world.
+// A persistent grid: each workgroup loads its rows of the DFT matrices once and takes
+// tiles blockIdx.x, + gridDim.x, ...; the next tile's elements are loaded into
+// registers while the current tile's products run.  Tiles are disjoint sets of lines,
+// so an in-place pass never reads a line another tile writes.
 template <int N, typename T, bool SPLIT, int AXIS, int IO>
 __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2c(PhaseArgs a, int nf) {
     using G = GC<N>;
     __shared__ __attribute__((aligned(16))) unsigned short sb[4][G::L * G::LDK];
-    const int tile = blockIdx.x % G::CTILES;
-    const int f = (blockIdx.x / G::CTILES) % nf;
-    const int wi = blockIdx.x / (G::CTILES * nf);
-    const DevWorld &w = a.worlds[a.ids[wi]];
-    if (f >= (IO == 0 ? w.C : w.K)) return;
-    const float *src;
-    float *dst;
-    const float *kh = nullptr;
-    if (IO == 0) {
-        dst = a.pool + w.Fsp + f * G::SPEC;
-        src = dst;
-    } else if (IO == 1) {
-        src = a.pool + w.Fsp + w.kern[f].c0 * G::SPEC;
-        dst = a.pool + w.Gsp + f * G::SPEC;
-        kh = a.pool + w.KH + (long long)f * G::NB * G::NB * G::NB;
-    } else {
-        dst = a.pool + w.Gsp + f * G::SPEC;
-        src = dst;
-    }
+    const int total = a.nworlds * nf * G::CTILES;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // stage: thread -> (line l, chunk q of 8 consecutive j)
+    // tile id -> (world, field, tile); false for fields past the world's count
+    auto decode = [&](int id, const DevWorld *&w, int &f, int &tile) {
+        tile = id % G::CTILES;
+        f = (id / G::CTILES) % nf;
+        w = &a.worlds[a.ids[id / (G::CTILES * nf)]];
+        return f < (IO == 0 ? w->C : w->K);
+    };
+    auto next_valid = [&](int id) {
+        const DevWorld *w;
+        int f, tile;
+        while (id < total && !decode(id, w, f, tile)) id += gridDim.x;
+        return id;
+    };
+    // staged elements of this thread: (line l, chunk q of 8 consecutive j) per round
+    float pre_re[G::ITER][8], pre_im[G::ITER][8], pre_m[G::ITER][8];
+    auto fetch = [&](int id) {
+        const DevWorld *w;
+        int f, tile;
+        decode(id, w, f, tile);
+        const float *src = a.pool + (IO == 1 ? w->Fsp + w->kern[f].c0 * G::SPEC : (IO == 0 ? w->Fsp : w->Gsp) + f * G::SPEC);
+        const float *kh = a.pool + w->KH + (long long)f * G::NB * G::NB * G::NB;
 #pragma unroll
-    for (int it = 0; it < G::ITER; it++) {
-        const int l = tid & 31, q = (tid >> 5) + it * (G::THREADS / 32);
-        const int line = tile * G::L + l, outer = line / G::NB, kx = line % G::NB;
-        float re[8], im[8];
+        for (int it = 0; it < G::ITER; it++) {
+            const int l = tid & 31, q = (tid >> 5) + it * (G::THREADS / 32);
+            const int line = tile * G::L + l, outer = line / G::NB, kx = line % G::NB;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int j = q * 8 + i;
-            const long long row = AXIS == 1 ? (long long)outer * N + j : (long long)j * N + outer;
-            re[i] = src[row * G::RS + kx];
-            im[i] = src[row * G::RS + G::NB + kx];
-            if (IO == 1) {
-                const float m = kh[((long long)fold(j, N) * G::NB + fold(outer, N)) * G::NB + kx];
-                re[i] = re[i] * m;
-                im[i] = im[i] * m;
+            for (int i = 0; i < 8; i++) {
+                const int j = q * 8 + i;
+                const long long row = AXIS == 1 ? (long long)outer * N + j : (long long)j * N + outer;
+                pre_re[it][i] = src[row * G::RS + kx];
+                pre_im[it][i] = src[row * G::RS + G::NB + kx];
+                if (IO == 1) pre_m[it][i] = kh[((long long)fold(j, N) * G::NB + fold(outer, N)) * G::NB + kx];
             }
         }
-        u16x8 h, lo;
-        split8<T>(re, h, lo);
-        sts8(&sb[0][l * G::LDK + q * 8], h);
-        if (SPLIT) sts8(&sb[1][l * G::LDK + q * 8], lo);
-        split8<T>(im, h, lo);
-        sts8(&sb[2][l * G::LDK + q * 8], h);
-        if (SPLIT) sts8(&sb[3][l * G::LDK + q * 8], lo);
-    }
+    };
+    int id = next_valid(blockIdx.x);
+    if (id >= total) return;
     u16x8 rh[G::RB][G::KS], rl[G::RB][G::KS], ih[G::RB][G::KS], il[G::RB][G::KS];
     const int rb0 = wave * G::RB;
     load_matrix<N, SPLIT>(a.dft, IO == 0 ? M_FR : M_GR, rb0, lane, rh, rl);
     load_matrix<N, SPLIT>(a.dft, IO == 0 ? M_FI : M_GI, rb0, lane, ih, il);
-    __syncthreads();
-    f32x4 cr[G::RB][G::LB], ci[G::RB][G::LB], xr[G::RB][G::LB], xi[G::RB][G::LB];
+    fetch(id);
+    for (;;) {
+        __syncthreads();  // the previous tile's products have read the staged tile
 #pragma unroll
-    for (int rb = 0; rb < G::RB; rb++)
+        for (int it = 0; it < G::ITER; it++) {
+            const int l = tid & 31, q = (tid >> 5) + it * (G::THREADS / 32);
+            float re[8], im[8];
 #pragma unroll
-        for (int lb = 0; lb < G::LB; lb++) {
-            cr[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            ci[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            xr[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            xi[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < 8; i++) {
+                re[i] = IO == 1 ? pre_re[it][i] * pre_m[it][i] : pre_re[it][i];
+                im[i] = IO == 1 ? pre_im[it][i] * pre_m[it][i] : pre_im[it][i];
+            }
+            u16x8 h, lo;
+            split8<T>(re, h, lo);
+            sts8(&sb[0][l * G::LDK + q * 8], h);
+            if (SPLIT) sts8(&sb[1][l * G::LDK + q * 8], lo);
+            split8<T>(im, h, lo);
+            sts8(&sb[2][l * G::LDK + q * 8], h);
+            if (SPLIT) sts8(&sb[3][l * G::LDK + q * 8], lo);
         }
+        const int cur = id;
+        id = next_valid(id + gridDim.x);
+        if (id < total) fetch(id);  // in flight during this tile's products
+        __syncthreads();
+        f32x4 cr[G::RB][G::LB], ci[G::RB][G::LB], xr[G::RB][G::LB], xi[G::RB][G::LB];
 #pragma unroll
-    for (int ks = 0; ks < G::KS; ks++)
+        for (int rb = 0; rb < G::RB; rb++)
 #pragma unroll
-        for (int lb = 0; lb < G::LB; lb++) {
-            const int off = (lb * 16 + (lane & 15)) * G::LDK + ks * 32 + 8 * (lane >> 4);
-            const u16x8 xrh = lds8(&sb[0][off]), xih = lds8(&sb[2][off]);
-            const u16x8 nih = xih ^ (unsigned short)0x8000;
-            u16x8 xrl, xil, nil;
-            if (SPLIT) {
-                xrl = lds8(&sb[1][off]);
-                xil = lds8(&sb[3][off]);
-                nil = xil ^ (unsigned short)0x8000;
+            for (int lb = 0; lb < G::LB; lb++) {
+                cr[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                ci[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                xr[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                xi[rb][lb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             }
 #pragma unroll
-            for (int rb = 0; rb < G::RB; rb++) {
-                // (Mr + i Mi)(Xr + i Xi): re = Mr Xr - Mi Xi, im = Mr Xi + Mi Xr
-                cr[rb][lb] = MT<T>::mac(rh[rb][ks], xrh, cr[rb][lb]);
-                cr[rb][lb] = MT<T>::mac(ih[rb][ks], nih, cr[rb][lb]);
-                ci[rb][lb] = MT<T>::mac(rh[rb][ks], xih, ci[rb][lb]);
-                ci[rb][lb] = MT<T>::mac(ih[rb][ks], xrh, ci[rb][lb]);
+        for (int ks = 0; ks < G::KS; ks++)
+#pragma unroll
+            for (int lb = 0; lb < G::LB; lb++) {
+                const int off = (lb * 16 + (lane & 15)) * G::LDK + ks * 32 + 8 * (lane >> 4);
+                const u16x8 xrh = lds8(&sb[0][off]), xih = lds8(&sb[2][off]);
+                const u16x8 nih = xih ^ (unsigned short)0x8000;
+                u16x8 xrl, xil, nil;
                 if (SPLIT) {
-                    xr[rb][lb] = MT<T>::mac(rh[rb][ks], xrl, xr[rb][lb]);
-                    xr[rb][lb] = MT<T>::mac(rl[rb][ks], xrh, xr[rb][lb]);
-                    xr[rb][lb] = MT<T>::mac(ih[rb][ks], nil, xr[rb][lb]);
-                    xr[rb][lb] = MT<T>::mac(il[rb][ks], nih, xr[rb][lb]);
-                    xi[rb][lb] = MT<T>::mac(rh[rb][ks], xil, xi[rb][lb]);
-                    xi[rb][lb] = MT<T>::mac(rl[rb][ks], xih, xi[rb][lb]);
-                    xi[rb][lb] = MT<T>::mac(ih[rb][ks], xrl, xi[rb][lb]);
-                    xi[rb][lb] = MT<T>::mac(il[rb][ks], xrh, xi[rb][lb]);
+                    xrl = lds8(&sb[1][off]);
+                    xil = lds8(&sb[3][off]);
+                    nil = xil ^ (unsigned short)0x8000;
+                }
+#pragma unroll
+                for (int rb = 0; rb < G::RB; rb++) {
+                    // (Mr + i Mi)(Xr + i Xi): re = Mr Xr - Mi Xi, im = Mr Xi + Mi Xr
+                    cr[rb][lb] = MT<T>::mac(rh[rb][ks], xrh, cr[rb][lb]);
+                    cr[rb][lb] = MT<T>::mac(ih[rb][ks], nih, cr[rb][lb]);
+                    ci[rb][lb] = MT<T>::mac(rh[rb][ks], xih, ci[rb][lb]);
+                    ci[rb][lb] = MT<T>::mac(ih[rb][ks], xrh, ci[rb][lb]);
+                    if (SPLIT) {
+                        xr[rb][lb] = MT<T>::mac(rh[rb][ks], xrl, xr[rb][lb]);
+                        xr[rb][lb] = MT<T>::mac(rl[rb][ks], xrh, xr[rb][lb]);
+                        xr[rb][lb] = MT<T>::mac(ih[rb][ks], nil, xr[rb][lb]);
+                        xr[rb][lb] = MT<T>::mac(il[rb][ks], nih, xr[rb][lb]);
+                        xi[rb][lb] = MT<T>::mac(rh[rb][ks], xil, xi[rb][lb]);
+                        xi[rb][lb] = MT<T>::mac(rl[rb][ks], xih, xi[rb][lb]);
+                        xi[rb][lb] = MT<T>::mac(ih[rb][ks], xrl, xi[rb][lb]);
+                        xi[rb][lb] = MT<T>::mac(il[rb][ks], xrh, xi[rb][lb]);
+                    }
                 }
             }
-        }
-    // every line of this tile was read before the barrier; write the transformed lines
-    const float scale = IO == 0 ? 1.0f / N : 1.0f;
+        const DevWorld *w;
+        int f, tile;
+        decode(cur, w, f, tile);
+        float *dst = a.pool + (IO == 0 ? w->Fsp : w->Gsp) + f * G::SPEC;
+        const float scale = IO == 0 ? 1.0f / N : 1.0f;
 #pragma unroll
-    for (int rb = 0; rb < G::RB; rb++)
+        for (int rb = 0; rb < G::RB; rb++)
 #pragma unroll
-        for (int lb = 0; lb < G::LB; lb++) {
-            const int line = tile * G::L + lb * 16 + (lane & 15), outer = line / G::NB, kx = line % G::NB;
+            for (int lb = 0; lb < G::LB; lb++) {
+                const int line = tile * G::L + lb * 16 + (lane & 15), outer = line / G::NB, kx = line % G::NB;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int k = (rb0 + rb) * 16 + 4 * (lane >> 4) + i;
-                const long long row = AXIS == 1 ? (long long)outer * N + k : (long long)k * N + outer;
-                const float re = SPLIT ? cr[rb][lb][i] + xr[rb][lb][i] * MT<T>::UNLO : cr[rb][lb][i];
-                const float im = SPLIT ? ci[rb][lb][i] + xi[rb][lb][i] * MT<T>::UNLO : ci[rb][lb][i];
-                dst[row * G::RS + kx] = re * scale;
-                dst[row * G::RS + G::NB + kx] = im * scale;
+                for (int i = 0; i < 4; i++) {
+                    const int k = (rb0 + rb) * 16 + 4 * (lane >> 4) + i;
+                    const long long row = AXIS == 1 ? (long long)outer * N + k : (long long)k * N + outer;
+                    const float re = SPLIT ? cr[rb][lb][i] + xr[rb][lb][i] * MT<T>::UNLO : cr[rb][lb][i];
+                    const float im = SPLIT ? ci[rb][lb][i] + xi[rb][lb][i] * MT<T>::UNLO : ci[rb][lb][i];
+                    dst[row * G::RS + kx] = re * scale;
+                    dst[row * G::RS + G::NB + kx] = im * scale;
+                }
             }
-        }
+        if (id >= total) break;
+    }
 }
 
 // Forward real transform along x of 32 rows of A_c into half-spectrum rows of Fsp[c].
@@ -440,6 +465,24 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2r_growth(PhaseArgs a, 
     }
 }
 
+// A persistent kernel over `total` tiles: as many workgroups as are resident at once (at
+// most one per tile); blocks per CU and CUs are queried once per kernel.
+template <class K>
+void launch_persistent(K *fn, int threads, int total, void *st, const PhaseArgs &a, int arg) {
+    static const int slots = [fn, threads] {
+        int nb = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(fn), threads, 0) !=
+                hipSuccess || nb < 1)
+            nb = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        return nb * cus;
+    }();
+    const int g = total < slots ? total : slots;
+    if (g > 0) hipLaunchKernelGGL(fn, dim3(g), dim3(threads), 0, (hipStream_t)st, a, arg);
+}
+
 template <int N, typename T, bool SPLIT>
 struct GLaunch {
     using G = GC<N>;
@@ -448,20 +491,16 @@ struct GLaunch {
                            (hipStream_t)st, a, nf);
     }
     static void fy(const PhaseArgs &a, int nf, void *st) {
-        hipLaunchKernelGGL((k_dft_c2c<N, T, SPLIT, 1, 0>), dim3(a.nworlds * nf * G::CTILES), dim3(G::THREADS), 0,
-                           (hipStream_t)st, a, nf);
+        launch_persistent(k_dft_c2c<N, T, SPLIT, 1, 0>, G::THREADS, a.nworlds * nf * G::CTILES, st, a, nf);
     }
     static void fz(const PhaseArgs &a, int nf, void *st) {
-        hipLaunchKernelGGL((k_dft_c2c<N, T, SPLIT, 2, 0>), dim3(a.nworlds * nf * G::CTILES), dim3(G::THREADS), 0,
-                           (hipStream_t)st, a, nf);
+        launch_persistent(k_dft_c2c<N, T, SPLIT, 2, 0>, G::THREADS, a.nworlds * nf * G::CTILES, st, a, nf);
     }
     static void iz(const PhaseArgs &a, int nf, void *st) {
-        hipLaunchKernelGGL((k_dft_c2c<N, T, SPLIT, 2, 1>), dim3(a.nworlds * nf * G::CTILES), dim3(G::THREADS), 0,
-                           (hipStream_t)st, a, nf);
+        launch_persistent(k_dft_c2c<N, T, SPLIT, 2, 1>, G::THREADS, a.nworlds * nf * G::CTILES, st, a, nf);
     }
     static void iy(const PhaseArgs &a, int nf, void *st) {
-        hipLaunchKernelGGL((k_dft_c2c<N, T, SPLIT, 1, 2>), dim3(a.nworlds * nf * G::CTILES), dim3(G::THREADS), 0,
-                           (hipStream_t)st, a, nf);
+        launch_persistent(k_dft_c2c<N, T, SPLIT, 1, 2>, G::THREADS, a.nworlds * nf * G::CTILES, st, a, nf);
     }
     static void ix(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_dft_c2r_growth<N, T, SPLIT>), dim3(a.nworlds * G::RTILES), dim3(G::THREADS), 0,
