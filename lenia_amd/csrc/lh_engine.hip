@@ -208,17 +208,20 @@ void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg)
         fn(a, arg, e.stream);
         return;
     }
-    hipEvent_t t0, t1;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
-    hipEventRecord(t0, e.stream);
-    fn(a, arg, e.stream);
-    hipEventRecord(t1, e.stream);
-    hipEventSynchronize(t1);
-    float ms = 0.0f;
-    hipEventElapsedTime(&ms, t0, t1);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float ms = -1.0f;
+    if (hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess &&
+        hipEventRecord(t0, e.stream) == hipSuccess) {
+        fn(a, arg, e.stream);
+        if (hipEventRecord(t1, e.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+            hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+            ms = -1.0f;
+    } else {
+        fn(a, arg, e.stream);  // launched untimed
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (ms < 0.0f) return;  // no timing recorded for this launch
     const std::string key = name;
     auto it = e.prof.find(key);
     if (it == e.prof.end()) {
@@ -379,19 +382,20 @@ int64_t field_off(const Engine &e, const HostWorld &w, int field) {
 
 // Average time of reps launches of one phase (after one untimed launch).
 float time_phase(Engine &e, launch_fn fn, const PhaseArgs &a, int arg, int reps) {
-    hipEvent_t t0, t1;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
-    fn(a, arg, e.stream);
-    hipEventRecord(t0, e.stream);
-    for (int r = 0; r < reps; r++) fn(a, arg, e.stream);
-    hipEventRecord(t1, e.stream);
-    hipEventSynchronize(t1);
-    float ms = 0.0f;
-    hipEventElapsedTime(&ms, t0, t1);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    return ms / reps;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float ms = -1.0f;
+    if (hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess) {
+        fn(a, arg, e.stream);
+        if (hipEventRecord(t0, e.stream) == hipSuccess) {
+            for (int r = 0; r < reps; r++) fn(a, arg, e.stream);
+            if (hipEventRecord(t1, e.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+                ms = -1.0f;
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    return ms < 0.0f ? 1e30f : ms / reps;  // a candidate that could not be timed is never picked
 }
 
 // Schedule autotuner: per shape class, times the gradient sweep at several segment
@@ -502,7 +506,7 @@ lh_status build_kernel_spectra(Engine &e) {
 void destroy_graphs(Engine &e) {
     for (int i = 0; i < 2; i++)
         if (e.graph[i]) {
-            hipGraphExecDestroy(e.graph[i]);
+            (void)hipGraphExecDestroy(e.graph[i]);
             e.graph[i] = nullptr;
         }
 }
@@ -544,25 +548,25 @@ lh_status lh_create(int device, void **out) {
 lh_status lh_destroy(void *p) {
     Engine *e = as_engine(p);
     if (!e) return LH_OK;
-    hipSetDevice(e->device);
-    hipStreamSynchronize(e->stream);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
     destroy_graphs(*e);
     for (ShapeClass &c : e->classes) {
-        hipFree(c.d_ids);
-        hipFree(c.d_dft);
+        (void)hipFree(c.d_ids);
+        (void)hipFree(c.d_dft);
     }
-    hipFree(e->pool);
-    hipFree(e->d_worlds);
-    hipFree(e->d_tw);
-    hipFree(e->d_mass);
-    hipFree(e->d_mom);
-    hipFree(e->d_mc);
+    (void)hipFree(e->pool);
+    (void)hipFree(e->d_worlds);
+    (void)hipFree(e->d_tw);
+    (void)hipFree(e->d_mass);
+    (void)hipFree(e->d_mom);
+    (void)hipFree(e->d_mc);
     for (Snapshot &sn : e->snap) {
-        hipFree(sn.d);
-        if (sn.done) hipEventDestroy(sn.done);
+        (void)hipFree(sn.d);
+        if (sn.done) (void)hipEventDestroy(sn.done);
     }
-    if (e->rb_stream) hipStreamDestroy(e->rb_stream);
-    hipStreamDestroy(e->stream);
+    if (e->rb_stream) (void)hipStreamDestroy(e->rb_stream);
+    (void)hipStreamDestroy(e->stream);
     delete e;
     return LH_OK;
 }
@@ -1085,7 +1089,7 @@ lh_status lh_mc_case(void *p, int world, int field, int channel, float iso, int 
     const size_t N = (size_t)hw.shape.W * hw.shape.H * hw.shape.D;
     HIPCHK(hipSetDevice(e->device));
     if (e->mc_cap < N) {
-        hipFree(e->d_mc);
+        HIPCHK(hipFree(e->d_mc));
         e->d_mc = nullptr;
         HIPCHK(hipMalloc(&e->d_mc, N));
         e->mc_cap = N;
@@ -1134,14 +1138,16 @@ lh_status lh_calibration_copy(void *p, size_t bytes) {
     void *a = nullptr, *b = nullptr;
     HIPCHK(hipMalloc(&a, bytes));
     if (hipMalloc(&b, bytes) != hipSuccess) {
-        hipFree(a);
+        (void)hipFree(a);
         return fail(LH_ENOMEM, "hipMalloc failed");
     }
-    hipMemsetAsync(a, 0, bytes, e->stream);
-    lh_launch_copy(a, b, bytes, e->stream);
-    hipError_t err = hipStreamSynchronize(e->stream);
-    hipFree(a);
-    hipFree(b);
+    hipError_t err = hipMemsetAsync(a, 0, bytes, e->stream);
+    if (err == hipSuccess) {
+        lh_launch_copy(a, b, bytes, e->stream);
+        err = hipStreamSynchronize(e->stream);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
     if (err != hipSuccess) return fail(LH_EHIP, hipGetErrorString(err));
     return LH_OK;
 }
