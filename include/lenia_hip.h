@@ -160,6 +160,34 @@ lh_status lh_world_moments(void *engine, double *out, int n_worlds);
 lh_status lh_mc_case(void *engine, int world, int field, int channel, float iso,
                      int face_only, uint8_t *out);
 
+/* ---- meshes ------------------------------------------------------------------------ */
+/* One mesh vertex: the reference's 28-byte Vertex (render.h:443-501) as the mesh
+ * threads fill it -- position, unnormalised accumulated normal, packed colour. */
+typedef struct lh_vertex {
+    float x, y, z, nx, ny, nz;
+    uint32_t color;
+} lh_vertex;
+
+/* This library's default triangulation table, tri[256 * 16], -1 terminated rows of
+ * edge triples (the layout of TRIANGLE_TABLE, marching_cubes.h:98-355), derived from
+ * the cube's topology.  Host only. */
+lh_status lh_mc_table(int *tri_table);
+
+/* Marching-cubes mesh of one world, replacing the mesh threads' generate_mesh_v1 /
+ * v2 / v3 (mesh.cpp:34, 434, 1115) for view 1 (A), 2 (A, U_SUM) or 3 (A, U_SUM, H),
+ * with one section over the whole grid.  Corner values are field > iso with the torus
+ * face pads of transpose_mesh_borders (zero edge/corner pads when face_only); corner
+ * positions are (g + delta) * cell_size.  tri_table: the caller's 256x16 table (the
+ * reference passes TRIANGLE_TABLE) or NULL for lh_mc_table's; colors[C]: the packed
+ * colour of each channel (NULL: white).  Vertices are numbered in the order the
+ * reference's single mesh thread creates them, indices follow its sweep, and normals
+ * are summed in its order, so the buffers match it bit for bit.  Writes the counts to
+ * *n_verts / *n_indices; fills verts / indices only when both fit max_verts /
+ * max_indices (pass NULL and 0 to query sizes).  View 2 and 3 need LH_OPT_KEEP_USUM. */
+lh_status lh_mesh(void *engine, int world, int view, float iso, float cell_size, int face_only,
+                  const int *tri_table, const uint32_t *colors, lh_vertex *verts, int64_t max_verts,
+                  uint32_t *indices, int64_t max_indices, int64_t *n_verts, int64_t *n_indices);
+
 /* ---- checkpoints ----------------------------------------------------------------- */
 /* One world's complete state as a flat, versioned byte image: the intent of the
  * reference's disabled save_config / load_config (config.cpp:75-261: the simulation

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblenia_hip.so")
-SOURCES = ["lh_kernels.hip", "lh_kernels_big.hip", "lh_dftgemm.hip", "lh_mlp.hip", "lh_sample.hip", "lh_engine.hip"]
+SOURCES = ["lh_kernels.hip", "lh_kernels_big.hip", "lh_dftgemm.hip", "lh_mlp.hip", "lh_sample.hip", "lh_mesh.hip",
+           "lh_engine.hip"]
 ARCH = os.environ.get("LENIA_HIP_ARCH", "gfx950")
 
 

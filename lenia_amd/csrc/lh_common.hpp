@@ -91,6 +91,29 @@ struct SeedJob {
 void lh_launch_seed_random(float *pool, const SeedJob *d_jobs, int n, int max_cells, uint64_t seed,
                            long long first_set, hipStream_t st);
 
+// Marching-cubes mesh of one world (lh_mesh): the view's fields (A; A, U_SUM; A, U_SUM,
+// H), each C channel planes of W*H*D floats.
+struct MeshJob {
+    const float *field[3];
+    int W, H, D, C, F, face_only;
+    float iso, cell;
+};
+// Device memory a mesh needs, grown on demand and kept by the engine.
+struct MeshBufs {
+    void *work = nullptr;
+    size_t work_cap = 0;
+    void *out = nullptr;
+    size_t out_cap = 0;
+};
+// The built-in triangulation table, tri[256 * 16] with -1 terminated rows.
+void lh_mc_derive(int *tri);
+// nullptr when tri is a usable triangulation table, else what is wrong with it.
+const char *lh_mc_invalid(const int *tri);
+// Counts the mesh of job (writes *nv, *ni) and, when verts and idx are given and the
+// counts fit max_v and max_i, emits it into those host buffers.  Synchronises st.
+hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *colors, MeshBufs &b, hipStream_t st,
+                       void *verts, int64_t max_v, uint32_t *idx, int64_t max_i, int64_t *nv, int64_t *ni);
+
 // Growth convolution as DFT-as-GEMM on matrix cores (cubic worlds of side N); arg of
 // fx/fy/fz: channels per world, of iz/iy: kernels per world.
 struct GemmOps {

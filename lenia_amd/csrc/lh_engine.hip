@@ -179,6 +179,7 @@ struct Engine {
     double *d_mom = nullptr;
     unsigned char *d_mc = nullptr;
     size_t mc_cap = 0;
+    MeshBufs mesh;
     int parity = 0;
     bool spectra_dirty = true;
     int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0, opt_autotune = 1;
@@ -561,6 +562,8 @@ lh_status lh_destroy(void *p) {
     (void)hipFree(e->d_mass);
     (void)hipFree(e->d_mom);
     (void)hipFree(e->d_mc);
+    (void)hipFree(e->mesh.work);
+    (void)hipFree(e->mesh.out);
     for (Snapshot &sn : e->snap) {
         (void)hipFree(sn.d);
         if (sn.done) (void)hipEventDestroy(sn.done);
@@ -1099,6 +1102,54 @@ lh_status lh_mc_case(void *p, int world, int field, int channel, float iso, int 
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, e->d_mc, N, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+lh_status lh_mc_table(int *tri) {
+    if (!tri) return fail(LH_EINVAL, "null argument");
+    lh_mc_derive(tri);
+    return LH_OK;
+}
+
+lh_status lh_mesh(void *p, int world, int view, float iso, float cell_size, int face_only, const int *tri_table,
+                  const uint32_t *colors, lh_vertex *verts, int64_t max_verts, uint32_t *indices, int64_t max_indices,
+                  int64_t *n_verts, int64_t *n_indices) {
+    Engine *e = as_engine(p);
+    if (!e || !n_verts || !n_indices) return fail(LH_EINVAL, "null argument");
+    if (max_verts < 0 || max_indices < 0) return fail(LH_EINVAL, "negative capacity");
+    if ((max_verts > 0 && !verts) || (max_indices > 0 && !indices)) return fail(LH_EINVAL, "null output buffer");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    if (view < 1 || view > 3) return fail(LH_EINVAL, "view must be 1 (A), 2 (A, U_SUM) or 3 (A, U_SUM, H)");
+    if (!std::isfinite(iso) || !std::isfinite(cell_size)) return fail(LH_EINVAL, "iso and cell_size must be finite");
+    HostWorld &hw = e->worlds[world];
+    if (view >= 2 && !hw.dev.keep_usum) return fail(LH_ESTATE, "views 2 and 3 need U_SUM: set LH_OPT_KEEP_USUM");
+    int table[256 * 16];
+    if (!tri_table) {
+        lh_mc_derive(table);
+        tri_table = table;
+    } else if (const char *why = lh_mc_invalid(tri_table)) {
+        return fail(LH_EINVAL, std::string("triangulation table: ") + why);
+    }
+    const int64_t N = (int64_t)hw.shape.W * hw.shape.H * hw.shape.D;
+    if (N * hw.shape.C * view * 15 >= ((int64_t)1 << 32))
+        return fail(LH_EUNSUPPORTED, "mesh of this world exceeds 32-bit vertex and index counts");
+    MeshJob job;
+    const int fields[3] = {LH_F_A, LH_F_USUM, LH_F_H};
+    for (int f = 0; f < 3; f++) job.field[f] = f < view ? e->pool + field_off(*e, hw, fields[f]) : nullptr;
+    job.W = hw.shape.W;
+    job.H = hw.shape.H;
+    job.D = hw.shape.D;
+    job.C = hw.shape.C;
+    job.F = view;
+    job.face_only = face_only ? 1 : 0;
+    job.iso = iso;
+    job.cell = cell_size;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(lh_mesh_run(job, tri_table, colors, e->mesh, e->stream, verts, max_verts, indices, max_indices, n_verts,
+                       n_indices));
+    if (verts && (*n_verts > max_verts || *n_indices > max_indices))
+        return fail(LH_EINVAL, "mesh does not fit max_verts / max_indices (counts written)");
     return LH_OK;
 }
 

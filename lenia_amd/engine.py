@@ -188,6 +188,31 @@ class Engine:
                                    out.ctypes.data_as(C.POINTER(C.c_uint8))), "lh_mc_case")
         return out.reshape(D, H, W)
 
+    def mesh(self, wid, view=1, iso=0.5, cell_size=3.5, face_only=True, tri_table=None, colors=None):
+        """Marching-cubes mesh of world wid for view 1 (A), 2 (A, U_SUM) or 3 (A, U_SUM, H), as
+        the renderer's mesh threads build it (mesh.cpp:34, 434, 1115): returns (vertices as
+        VERTEX_DTYPE records, uint32 indices).  tri_table: 256x16 ints, default mc_table()."""
+        tri = None
+        if tri_table is not None:
+            tri = np.ascontiguousarray(tri_table, dtype=np.int32).reshape(256, 16)
+        col = None
+        if colors is not None:
+            col = np.ascontiguousarray(colors, dtype=np.uint32)
+            if col.size < self.worlds[wid][1]:
+                raise ValueError("one colour per channel")
+        tp = tri.ctypes.data_as(C.POINTER(C.c_int)) if tri is not None else None
+        cp = col.ctypes.data_as(C.POINTER(C.c_uint32)) if col is not None else None
+        nv, ni = C.c_int64(), C.c_int64()
+        args = (self._h, wid, int(view), float(iso), float(cell_size), int(face_only), tp, cp)
+        L.check(self._L.lh_mesh(*args, None, 0, None, 0, C.byref(nv), C.byref(ni)), "lh_mesh")
+        verts = np.zeros(nv.value, dtype=VERTEX_DTYPE)
+        idx = np.zeros(ni.value, dtype=np.uint32)
+        if nv.value:
+            L.check(self._L.lh_mesh(*args, verts.ctypes.data_as(C.c_void_p), nv.value,
+                                    idx.ctypes.data_as(C.POINTER(C.c_uint32)), ni.value, C.byref(nv), C.byref(ni)),
+                    "lh_mesh")
+        return verts, idx
+
     def profile(self) -> dict:
         out = {}
         for i in range(self._L.lh_profile_count(self._h)):
@@ -206,6 +231,18 @@ class Engine:
 
 def shape_supported(W, H, D) -> bool:
     return bool(L.load().lh_shape_supported(W, H, D))
+
+
+VERTEX_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("nx", "<f4"), ("ny", "<f4"),
+                         ("nz", "<f4"), ("color", "<u4")])  # lh_vertex, the reference's 28-byte Vertex
+
+
+def mc_table() -> np.ndarray:
+    """The library's default marching-cubes triangulation table, int32 [256][16] with -1
+    terminated rows of edge triples (host code, no GPU needed)."""
+    out = np.empty((256, 16), dtype=np.int32)
+    L.check(L.load().lh_mc_table(out.ctypes.data_as(C.POINTER(C.c_int))), "lh_mc_table")
+    return out
 
 
 def sample_random_raw(seed, first, n, channels, device=0):

@@ -121,6 +121,21 @@ double lo_canon_exp(double x);
 /* Edge mask of a marching-cubes case, generated from the cube's edge list. */
 int lo_edge_mask(int case_index);
 
+/* One mesh vertex, the 28-byte Vertex of render.h:443-501: position, accumulated
+ * (unnormalised) normal, packed colour. */
+typedef struct lo_vertex {
+    float x, y, z, nx, ny, nz;
+    uint32_t color;
+} lo_vertex;
+
+/* Marching-cubes mesh of F fields (f0 = A, f1 = U_SUM, f2 = H; each C channels of
+ * W*H*D) after generate_mesh_v1/v2/v3 (lenia_mesh_oracle.c).  tri is a 256x16
+ * triangulation table (-1 terminated rows), colors[C].  Returns 0, or -1 when the
+ * counts exceed max_verts / max_indices (counts still written), -2 out of memory. */
+int lo_mesh(int W, int H, int D, int C, int F, const float *f0, const float *f1, const float *f2, float iso,
+            float cell, int face_only, const int *tri, const uint32_t *colors, lo_vertex *verts, int64_t max_verts,
+            uint32_t *indices, int64_t max_indices, int64_t *n_verts, int64_t *n_indices);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,11 @@ def lib():
         L.lo_edge_mask.restype = C.c_int
         L.lo_canon_exp.restype = C.c_double
         L.lo_canon_exp.argtypes = [C.c_double]
+        FP = C.POINTER(C.c_float)
+        L.lo_mesh.argtypes = ([C.c_int] * 5 + [FP] * 3 + [C.c_float, C.c_float, C.c_int, C.POINTER(C.c_int),
+                              C.POINTER(C.c_uint32), C.c_void_p, C.c_int64, C.POINTER(C.c_uint32), C.c_int64,
+                              C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        L.lo_mesh.restype = C.c_int
         _lib = L
     return _lib
 
@@ -260,6 +265,36 @@ def mc_case(field: np.ndarray, iso=0.5, face_only=True, prec=32):
     getattr(lib(), pfx + "mc_case")(W, H, D, _ptr(a, ct), ct(iso), int(face_only),
                                     out.ctypes.data_as(C.POINTER(C.c_uint8)))
     return out.reshape(D, H, W)
+
+
+VERTEX_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("nx", "<f4"), ("ny", "<f4"),
+                         ("nz", "<f4"), ("color", "<u4")])
+
+
+def mesh(fields, tri_table, iso=0.5, cell=3.5, face_only=True, colors=None):
+    """Marching-cubes mesh of 1-3 fields (A, U_SUM, H), each [C][D][H][W] float32, after
+    generate_mesh_v1/v2/v3 (mesh.cpp:34-2036) with one section: returns (vertices as
+    VERTEX_DTYPE records, uint32 indices)."""
+    F = len(fields)
+    assert 1 <= F <= 3
+    arrs = [np.ascontiguousarray(f, dtype=np.float32) for f in fields]
+    Cn, D, H, W = arrs[0].shape
+    tri = np.ascontiguousarray(tri_table, dtype=np.int32).reshape(256, 16)
+    col = np.ascontiguousarray(colors if colors is not None else [0xFFFFFFFF] * Cn, dtype=np.uint32)
+    FP = C.POINTER(C.c_float)
+    ptrs = [a.ctypes.data_as(FP) for a in arrs] + [None] * (3 - F)
+    nv, ni = C.c_int64(), C.c_int64()
+    L = lib()
+    r = L.lo_mesh(W, H, D, Cn, F, *ptrs, iso, cell, int(face_only), tri.ctypes.data_as(C.POINTER(C.c_int)),
+                  col.ctypes.data_as(C.POINTER(C.c_uint32)), None, 0, None, 0, C.byref(nv), C.byref(ni))
+    assert r in (0, -1), r
+    verts = np.zeros(nv.value, dtype=VERTEX_DTYPE)
+    idx = np.zeros(ni.value, dtype=np.uint32)
+    r = L.lo_mesh(W, H, D, Cn, F, *ptrs, iso, cell, int(face_only), tri.ctypes.data_as(C.POINTER(C.c_int)),
+                  col.ctypes.data_as(C.POINTER(C.c_uint32)), verts.ctypes.data_as(C.c_void_p), nv.value,
+                  idx.ctypes.data_as(C.POINTER(C.c_uint32)), ni.value, C.byref(nv), C.byref(ni))
+    assert r == 0, r
+    return verts, idx
 
 
 def canon_exp(x) -> np.ndarray:
