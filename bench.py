@@ -245,6 +245,42 @@ def sampler_leg(eng, n_worlds):
             "seed_worlds_per_s": n_worlds / t_seed, "seed_GBs_written": written / t_seed / 1e9}
 
 
+def mesh_leg(steps=4, reps=20):
+    """Marching-cubes meshes of the renderer's views (lh_mesh: vertices, indices and
+    normals on the GPU, copied into host buffers) for the reference's 32^3 x 3 world and
+    a 128^3 x 3 world, beside the single-threaded CPU restatement of the mesh thread on
+    the same fields (parity checked bit for bit)."""
+    from oracle import oracle as O
+    from lenia_amd.engine import mc_table
+    out = {}
+    tab = mc_table()
+    for n in (32, 128):
+        w = WL.world((n, n, n), 3, 9, 8, 300 + n)
+        eng = Engine(keep_usum=True)
+        wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        eng.set(wid, "A", w["A"])
+        eng.step(steps)
+        eng.synchronize()
+        fields = [eng.get(wid, f).reshape(3, n, n, n) for f in ("A", "USUM", "H")]
+        for view in (1, 3):
+            eng.mesh(wid, view=view)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                v, i = eng.mesh(wid, view=view)
+            t_gpu = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            rv, ri = O.mesh(fields[:view], tab)
+            t_cpu = time.perf_counter() - t0
+            out[f"{n}^3_view{view}"] = {
+                "vertices": len(v), "indices": len(i), "ms_per_mesh": t_gpu * 1e3,
+                "meshes_per_s": 1.0 / t_gpu,
+                "cpu_baseline": {"ms_per_mesh": t_cpu * 1e3, "cores": 1, "kind": "port"},
+                "bitwise_equal": bool(np.array_equal(i, ri) and v.tobytes() == rv.tobytes())}
+        eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +300,8 @@ def main():
                     help="skip timing checkpoint images (save and restore of 256 worlds)")
     ap.add_argument("--no-sampler", action="store_true",
                     help="skip timing the random-world sampler (10^6 sets) and seeding (every world)")
+    ap.add_argument("--no-mesh", action="store_true",
+                    help="skip timing the marching-cubes meshes (32^3 and 128^3, views 1 and 3)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -329,6 +367,9 @@ def main():
 
     if not args.no_proposal:
         line["proposal"] = proposal_leg()
+
+    if not args.no_mesh:
+        line["mesh"] = mesh_leg()
 
     if not args.no_precision:
         line["precision_128"] = precision_leg(args.steps, args.warmup)

@@ -7,16 +7,18 @@
 // order of "units" u = ((channel * cubes + cube) * F + field): a vertex belongs to the
 // unit and triangle slot that first names its edge, vertices are numbered in that order,
 // the index buffer runs in unit order, and a vertex's normal is the sum of the face
-// normals of the triangles naming it, in unit order.  Here every rule is data-parallel:
-//   count   one thread per unit: case index, triangle count, and an atomicMin of the
-//           64-bit key u * 16 + slot on each named edge (the first owner wins);
-//   own     per unit, the slots whose key won;
+// normals of the triangles naming it, in unit order.  Here every rule is data-parallel
+// (one thread per channel and cube walks that cube's F units, so that neighbouring
+// threads read neighbouring cells):
+//   count   case index, triangle count, and an atomicMin of the 64-bit key
+//           u * 16 + slot on each named edge (the first owner wins);
+//   own     the slots whose key won;
 //   scan    exclusive prefix sums of index and vertex counts over the units;
-//   emit    per unit: owned vertices at their ranks (position interpolated in the owning
-//           cube's corner order, as the reference stores it), edge -> vertex map;
-//   index   per unit: the index buffer;
-//   normal  per vertex: a gather over the (channel, adjacent cube, triangle) sequence that
-//           names its edge, summed in the reference's order, so the fp32 sums match.
+//   emit    owned vertices at their ranks (position interpolated in the owning cube's
+//           corner order, as the reference stores it), edge -> vertex map;
+//   index   the index buffer and each triangle's face normal at its serial position;
+//   normal  per vertex: a gather over the (channel, adjacent cube, triangle) sequence
+//           naming it, summed in the reference's order, so the fp32 sums match.
 // All interpolation and cross products are written as the reference's expressions
 // (no contraction: the build passes -ffp-contract=off).
 //
@@ -80,6 +82,7 @@ struct MeshDev {
 struct MeshOut {
     float *verts;     // [nv][7] (x, y, z, nx, ny, nz, color bits)
     uint32_t *idx;    // [ni]
+    float *fnorm;     // [ni / 3][3] face normal of each triangle
     uint32_t *vedge;  // [nv] edge slot (field * E3 + edge) of each vertex
 };
 
@@ -135,14 +138,17 @@ __device__ __forceinline__ void edge_point(const MeshDev &m, const float v[8], i
     pz = az + (bz - az) * T;
 }
 
-__device__ __forceinline__ void unit_coords(const MeshDev &m, long long u, int &c, int &f, int &x, int &y, int &z) {
-    f = (int)(u % m.j.F);
-    const long long t = u / m.j.F;
+// Thread t = channel * cubes + cube handles the F units t * F + f: consecutive threads
+// read consecutive cells of one field.
+__device__ __forceinline__ bool cube_coords(const MeshDev &m, long long &t, int &c, int &x, int &y, int &z) {
+    t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= m.U / m.j.F) return false;
     const long long cube = t % m.Nc;
     c = (int)(t / m.Nc);
     x = (int)(cube % m.j.W);
     y = (int)((cube / m.j.W) % m.j.H);
     z = (int)(cube / ((long long)m.j.W * m.j.H));
+    return true;
 }
 
 __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *p, long long n, unsigned long long v) {
@@ -150,43 +156,45 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *p, long lo
 }
 
 __global__ __launch_bounds__(256) void k_mesh_count(MeshDev m) {
-    const long long u = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (u >= m.U) return;
-    int c, f, x, y, z;
-    unit_coords(m, u, c, f, x, y, z);
-    float v[8];
-    load_corners(m, c, f, x, y, z, v);
-    const int cs = case_of(v, m.j.iso);
-    m.ucase[u] = (uint8_t)cs;
-    const int ns = 3 * m.ntri[cs];
-    m.cnt_i[u] = (uint32_t)ns;
-    const int8_t *row = m.tri + 16 * cs;
-    for (int s = 0; s < ns; s++) {
-        const long long g = f * m.E3 + edge_slot(m, x, y, z, row[s]);
-        atomicMin(m.first + g, (unsigned long long)u * 16ull + (unsigned long long)s);
+    long long t;
+    int c, x, y, z;
+    if (!cube_coords(m, t, c, x, y, z)) return;
+    for (int f = 0; f < m.j.F; f++) {
+        const long long u = t * m.j.F + f;
+        float v[8];
+        load_corners(m, c, f, x, y, z, v);
+        const int cs = case_of(v, m.j.iso);
+        m.ucase[u] = (uint8_t)cs;
+        const int ns = 3 * m.ntri[cs];
+        m.cnt_i[u] = (uint32_t)ns;
+        const int8_t *row = m.tri + 16 * cs;
+        for (int s = 0; s < ns; s++) {
+            const long long g = f * m.E3 + edge_slot(m, x, y, z, row[s]);
+            atomicMin(m.first + g, (unsigned long long)u * 16ull + (unsigned long long)s);
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_mesh_own(MeshDev m) {
-    const long long u = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (u >= m.U) return;
-    const int cs = m.ucase[u];
-    const int ns = 3 * m.ntri[cs];
-    uint32_t own = 0;
-    if (ns) {
-        int c, f, x, y, z;
-        unit_coords(m, u, c, f, x, y, z);
+    long long t;
+    int c, x, y, z;
+    if (!cube_coords(m, t, c, x, y, z)) return;
+    for (int f = 0; f < m.j.F; f++) {
+        const long long u = t * m.j.F + f;
+        const int cs = m.ucase[u];
+        const int ns = 3 * m.ntri[cs];
         const int8_t *row = m.tri + 16 * cs;
+        uint32_t own = 0;
         for (int s = 0; s < ns; s++) {
             const long long g = f * m.E3 + edge_slot(m, x, y, z, row[s]);
             own += m.first[g] == (unsigned long long)u * 16ull + (unsigned long long)s;
         }
+        m.cnt_v[u] = own;
     }
-    m.cnt_v[u] = own;
 }
 
 // ---- exclusive prefix sums of two uint32 arrays at once ---------------------------
-constexpr int SCAN_T = 256, SCAN_PER = 4, SCAN_TILE = SCAN_T * SCAN_PER;
+constexpr int SCAN_T = 256, SCAN_PER = 8, SCAN_TILE = SCAN_T * SCAN_PER;
 
 __device__ __forceinline__ uint2 block_exclusive(uint2 x, uint2 *lds, uint2 &total) {
     const int t = threadIdx.x;
@@ -204,17 +212,20 @@ __device__ __forceinline__ uint2 block_exclusive(uint2 x, uint2 *lds, uint2 &tot
     return make_uint2(incl.x - x.x, incl.y - x.y);
 }
 
+// Tile sums (order-free, so the loads run across the workgroup: item k * SCAN_T + t).
 __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint32_t *a, const uint32_t *b, long long n,
                                                        uint32_t *ta, uint32_t *tb) {
     __shared__ uint2 lds[SCAN_T];
-    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_PER;
+    const long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint2 s = make_uint2(0u, 0u);
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) {
-            s.x += a[base + k];
-            s.y += b[base + k];
+    for (int k = 0; k < SCAN_PER; k++) {
+        const long long i = base + (long long)k * SCAN_T;
+        if (i < n) {
+            s.x += a[i];
+            s.y += b[i];
         }
+    }
     uint2 total;
     (void)block_exclusive(s, lds, total);
     if (threadIdx.x == 0) {
@@ -244,78 +255,119 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_top(uint32_t *ta, uint32_t *tb,
     }
 }
 
+// In-place exclusive offsets of one tile: staged through LDS so that global loads and
+// stores run across the workgroup while each thread scans SCAN_PER consecutive items.
 __global__ __launch_bounds__(SCAN_T) void k_scan_apply(uint32_t *a, uint32_t *b, long long n, const uint32_t *ta,
                                                        const uint32_t *tb) {
     __shared__ uint2 lds[SCAN_T];
-    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_PER;
+    __shared__ uint32_t sa[SCAN_TILE], sb[SCAN_TILE];
+    const long long tile = (long long)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int j = k * SCAN_T + threadIdx.x;
+        sa[j] = tile + j < n ? a[tile + j] : 0u;
+        sb[j] = tile + j < n ? b[tile + j] : 0u;
+    }
+    __syncthreads();
     uint32_t va[SCAN_PER], vb[SCAN_PER];
     uint2 s = make_uint2(0u, 0u);
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++) {
-        va[k] = base + k < n ? a[base + k] : 0u;
-        vb[k] = base + k < n ? b[base + k] : 0u;
+        va[k] = sa[threadIdx.x * SCAN_PER + k];
+        vb[k] = sb[threadIdx.x * SCAN_PER + k];
         s.x += va[k];
         s.y += vb[k];
     }
     uint2 total;
-    uint2 run = block_exclusive(s, lds, total);
+    uint2 run = block_exclusive(s, lds, total);  // ends with a barrier: sa, sb reads done
     run.x += ta[blockIdx.x];
     run.y += tb[blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) {
-            a[base + k] = run.x;
-            b[base + k] = run.y;
-            run.x += va[k];
-            run.y += vb[k];
+    for (int k = 0; k < SCAN_PER; k++) {
+        sa[threadIdx.x * SCAN_PER + k] = run.x;
+        sb[threadIdx.x * SCAN_PER + k] = run.y;
+        run.x += va[k];
+        run.y += vb[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int j = k * SCAN_T + threadIdx.x;
+        if (tile + j < n) {
+            a[tile + j] = sa[j];
+            b[tile + j] = sb[j];
         }
-}
-
-__global__ __launch_bounds__(256) void k_mesh_emit(MeshDev m, MeshOut o) {
-    const long long u = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (u >= m.U) return;
-    const int cs = m.ucase[u];
-    const int ns = 3 * m.ntri[cs];
-    if (!ns) return;
-    int c, f, x, y, z;
-    unit_coords(m, u, c, f, x, y, z);
-    float v[8];
-    load_corners(m, c, f, x, y, z, v);
-    const int8_t *row = m.tri + 16 * cs;
-    uint32_t vid = m.cnt_v[u];
-    for (int s = 0; s < ns; s++) {
-        const int e = row[s];
-        const long long g = f * m.E3 + edge_slot(m, x, y, z, e);
-        if (m.first[g] != (unsigned long long)u * 16ull + (unsigned long long)s) continue;
-        float px, py, pz;
-        edge_point(m, v, x, y, z, e, px, py, pz);
-        float *out = o.verts + 7ll * vid;
-        out[0] = px;
-        out[1] = py;
-        out[2] = pz;
-        out[6] = __uint_as_float(m.colors[c]);
-        m.vmap[g] = vid;
-        o.vedge[vid] = (uint32_t)g;
-        vid++;
     }
 }
 
-__global__ __launch_bounds__(256) void k_mesh_index(MeshDev m, MeshOut o) {
-    const long long u = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (u >= m.U) return;
-    const int cs = m.ucase[u];
-    const int ns = 3 * m.ntri[cs];
-    if (!ns) return;
-    int c, f, x, y, z;
-    unit_coords(m, u, c, f, x, y, z);
-    const int8_t *row = m.tri + 16 * cs;
-    uint32_t *dst = o.idx + m.cnt_i[u];
-    for (int s = 0; s < ns; s++) dst[s] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, row[s])];
+__global__ __launch_bounds__(256) void k_mesh_emit(MeshDev m, MeshOut o) {
+    long long t;
+    int c, x, y, z;
+    if (!cube_coords(m, t, c, x, y, z)) return;
+    for (int f = 0; f < m.j.F; f++) {
+        const long long u = t * m.j.F + f;
+        const int cs = m.ucase[u];
+        const int ns = 3 * m.ntri[cs];
+        if (!ns) continue;
+        float v[8];
+        load_corners(m, c, f, x, y, z, v);
+        const int8_t *row = m.tri + 16 * cs;
+        uint32_t vid = m.cnt_v[u];
+        for (int s = 0; s < ns; s++) {
+            const int e = row[s];
+            const long long g = f * m.E3 + edge_slot(m, x, y, z, e);
+            if (m.first[g] != (unsigned long long)u * 16ull + (unsigned long long)s) continue;
+            float px, py, pz;
+            edge_point(m, v, x, y, z, e, px, py, pz);
+            float *out = o.verts + 7ll * vid;
+            out[0] = px;
+            out[1] = py;
+            out[2] = pz;
+            out[6] = __uint_as_float(m.colors[c]);
+            m.vmap[g] = vid;
+            o.vedge[vid] = (uint32_t)g;
+            vid++;
+        }
+    }
 }
 
-// Normal of vertex i: the face normals (mesh.cpp:229-300) of every triangle naming its
-// edge, summed over channels, then the (up to four) cubes around the edge in sweep
-// order, then the triangles of each cube in table order -- the reference's order.
+// The index buffer, and the face normal of every triangle (mesh.cpp:229-300) from the
+// cube's own interpolated corners, at the triangle's serial position.
+__global__ __launch_bounds__(256) void k_mesh_index(MeshDev m, MeshOut o) {
+    long long t;
+    int c, x, y, z;
+    if (!cube_coords(m, t, c, x, y, z)) return;
+    for (int f = 0; f < m.j.F; f++) {
+        const long long u = t * m.j.F + f;
+        const int cs = m.ucase[u];
+        const int nt = m.ntri[cs];
+        if (!nt) continue;
+        float v[8];
+        load_corners(m, c, f, x, y, z, v);
+        const int8_t *row = m.tri + 16 * cs;
+        const uint32_t base = m.cnt_i[u];
+        for (int tri = 0; tri < nt; tri++) {
+            const int e0 = row[3 * tri], e1 = row[3 * tri + 1], e2 = row[3 * tri + 2];
+            o.idx[base + 3 * tri] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e0)];
+            o.idx[base + 3 * tri + 1] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e1)];
+            o.idx[base + 3 * tri + 2] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e2)];
+            float ax, ay, az, bx, by, bz, qx, qy, qz;
+            edge_point(m, v, x, y, z, e0, ax, ay, az);
+            edge_point(m, v, x, y, z, e1, bx, by, bz);
+            edge_point(m, v, x, y, z, e2, qx, qy, qz);
+            const float ux = bx - ax, uy = by - ay, uz = bz - az;
+            const float vx = qx - ax, vy = qy - ay, vz = qz - az;
+            float *fn = o.fnorm + base + 3 * tri;
+            fn[0] = uy * vz - uz * vy;
+            fn[1] = uz * vx - ux * vz;
+            fn[2] = ux * vy - uy * vx;
+        }
+    }
+}
+
+// Normal of vertex i: the face normals of every triangle naming it, summed over
+// channels, then the (up to four) cubes around its edge in sweep order, then each cube's
+// triangles in table order -- the reference's order, so the fp32 sums agree.
 __global__ __launch_bounds__(256) void k_mesh_normal(MeshDev m, MeshOut o, long long nv) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= nv) return;
@@ -338,30 +390,14 @@ __global__ __launch_bounds__(256) void k_mesh_normal(MeshDev m, MeshOut o, long 
             if (cx < 0 || cy < 0 || cz < 0 || cx >= m.j.W || cy >= m.j.H || cz >= m.j.D) continue;
             const long long cube = cx + (long long)m.j.W * (cy + (long long)m.j.H * cz);
             const long long u = ((long long)c * m.Nc + cube) * m.j.F + f;
-            const int cs = m.ucase[u];
-            const int nt = m.ntri[cs];
-            const int8_t *row = m.tri + 16 * cs;
-            bool loaded = false;
-            float v[8];
+            const int nt = m.ntri[m.ucase[u]];
+            const uint32_t base = m.cnt_i[u];
             for (int tri = 0; tri < nt; tri++) {
-                const int e0 = row[3 * tri], e1 = row[3 * tri + 1], e2 = row[3 * tri + 2];
-                const long long l = g - (long long)f * m.E3;
-                const int hits = (edge_slot(m, cx, cy, cz, e0) == l) + (edge_slot(m, cx, cy, cz, e1) == l) +
-                                 (edge_slot(m, cx, cy, cz, e2) == l);
+                const uint32_t *ix = o.idx + base + 3 * tri;
+                const int hits = (ix[0] == (uint32_t)i) + (ix[1] == (uint32_t)i) + (ix[2] == (uint32_t)i);
                 if (!hits) continue;
-                if (!loaded) {
-                    load_corners(m, c, f, cx, cy, cz, v);
-                    loaded = true;
-                }
-                float ax, ay, az, bxp, byp, bzp, qx, qy, qz;
-                edge_point(m, v, cx, cy, cz, e0, ax, ay, az);
-                edge_point(m, v, cx, cy, cz, e1, bxp, byp, bzp);
-                edge_point(m, v, cx, cy, cz, e2, qx, qy, qz);
-                const float ux = bxp - ax, uy = byp - ay, uz = bzp - az;
-                const float vx = qx - ax, vy = qy - ay, vz = qz - az;
-                const float fx = uy * vz - uz * vy;
-                const float fy = uz * vx - ux * vz;
-                const float fz = ux * vy - uy * vx;
+                const float *fn = o.fnorm + base + 3 * tri;
+                const float fx = fn[0], fy = fn[1], fz = fn[2];
                 for (int h = 0; h < hits; h++) {
                     nx = nx + fx;
                     ny = ny + fy;
@@ -592,8 +628,8 @@ hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *color
     const long long nfirst = job.F * m.E3;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)std::min<long long>(grid_for(nfirst), 4096)), dim3(256), 0, st,
                        m.first, nfirst, ~0ull);
-    hipLaunchKernelGGL(k_mesh_count, dim3(grid_for(m.U)), dim3(256), 0, st, m);
-    hipLaunchKernelGGL(k_mesh_own, dim3(grid_for(m.U)), dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_mesh_count, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_mesh_own, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m);
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.cnt_i, m.cnt_v, m.U, ta, tb);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, ta, tb, ntiles, tot);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.cnt_i, m.cnt_v, m.U, ta, tb);
@@ -605,13 +641,14 @@ hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *color
     *nv = htot[1];
     if (!verts || !idx || *nv > max_v || *ni > max_i || *nv == 0) return hipSuccess;
     const size_t vb = up256((size_t)*nv * 28), ib = up256((size_t)*ni * 4), eb = up256((size_t)*nv * 4);
-    if ((e = grow(&b.out, &b.out_cap, vb + ib + eb)) != hipSuccess) return e;
+    if ((e = grow(&b.out, &b.out_cap, vb + 2 * ib + eb)) != hipSuccess) return e;
     MeshOut o;
     o.verts = (float *)b.out;
     o.idx = (uint32_t *)((char *)b.out + vb);
-    o.vedge = (uint32_t *)((char *)b.out + vb + ib);
-    hipLaunchKernelGGL(k_mesh_emit, dim3(grid_for(m.U)), dim3(256), 0, st, m, o);
-    hipLaunchKernelGGL(k_mesh_index, dim3(grid_for(m.U)), dim3(256), 0, st, m, o);
+    o.fnorm = (float *)((char *)b.out + vb + ib);
+    o.vedge = (uint32_t *)((char *)b.out + vb + 2 * ib);
+    hipLaunchKernelGGL(k_mesh_emit, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m, o);
+    hipLaunchKernelGGL(k_mesh_index, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m, o);
     hipLaunchKernelGGL(k_mesh_normal, dim3(grid_for(*nv)), dim3(256), 0, st, m, o, (long long)*nv);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(verts, o.verts, (size_t)*nv * 28, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;

@@ -204,14 +204,26 @@ class Engine:
         cp = col.ctypes.data_as(C.POINTER(C.c_uint32)) if col is not None else None
         nv, ni = C.c_int64(), C.c_int64()
         args = (self._h, wid, int(view), float(iso), float(cell_size), int(face_only), tp, cp)
-        L.check(self._L.lh_mesh(*args, None, 0, None, 0, C.byref(nv), C.byref(ni)), "lh_mesh")
-        verts = np.zeros(nv.value, dtype=VERTEX_DTYPE)
-        idx = np.zeros(ni.value, dtype=np.uint32)
-        if nv.value:
-            L.check(self._L.lh_mesh(*args, verts.ctypes.data_as(C.c_void_p), nv.value,
-                                    idx.ctypes.data_as(C.POINTER(C.c_uint32)), ni.value, C.byref(nv), C.byref(ni)),
-                    "lh_mesh")
-        return verts, idx
+        # one pass when the buffers of the last mesh of this size class are large enough,
+        # else a second pass into buffers of the counts the first one reported
+        key = (wid, int(view))
+        cap_v, cap_i = getattr(self, "_mesh_caps", {}).get(key, (0, 0))
+        for _ in range(2):
+            verts = np.zeros(cap_v, dtype=VERTEX_DTYPE)
+            idx = np.zeros(cap_i, dtype=np.uint32)
+            rc = self._L.lh_mesh(*args, verts.ctypes.data_as(C.c_void_p) if cap_v else None, cap_v,
+                                 idx.ctypes.data_as(C.POINTER(C.c_uint32)) if cap_i else None, cap_i,
+                                 C.byref(nv), C.byref(ni))
+            fits = nv.value <= cap_v and ni.value <= cap_i
+            if rc == L.OK and fits and (cap_v or not nv.value):
+                break
+            if fits:
+                L.check(rc, "lh_mesh")
+            cap_v, cap_i = max(cap_v, int(nv.value * 1.25) + 64), max(cap_i, int(ni.value * 1.25) + 192)
+        if not hasattr(self, "_mesh_caps"):
+            self._mesh_caps = {}
+        self._mesh_caps[key] = (cap_v, cap_i)
+        return verts[:nv.value], idx[:ni.value]
 
     def profile(self) -> dict:
         out = {}

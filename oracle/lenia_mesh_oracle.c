@@ -1,8 +1,8 @@
 /* Marching-cubes meshes of the renderer's views -- CPU restatement, TEST INFRASTRUCTURE.
  *
  * Follows generate_mesh_v1 / v2 / v3 (mesh.cpp:34-430, 434-1113, 1115-2036) with one
- * mesh section spanning the whole grid (MESH_THREADS = 1 in render.h:15, so the border
- * stitching of main.cpp:104-178 has nothing to join):
+ * mesh section spanning the whole grid (as if MESH_THREADS, render.h:15, were 1, so the border
+ * stitching of main.cpp:105-170 has nothing to join):
  *   for channel c, for cube z, y, x, for view field f (1: A; 2: A, U_SUM; 3: A, U_SUM, H):
  *     case index from the 8 corner values > iso (mesh.cpp:173-180);
  *     each crossing edge interpolated in the cube's own corner order (mesh.cpp:185-201):

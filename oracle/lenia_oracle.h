@@ -25,6 +25,10 @@
  *                          physics mode, brute-force box-overlap integral on 4^3.
  *   MC case index          pinned: brute force + generated edge table vs the reference's
  *                          EDGE_TABLE fixture (tests/golden/edge_table.json).
+ *   mesh (lo_mesh)         pinned: sphere / torus Euler characteristic 2 / 0, closed and
+ *                          consistently oriented, outward normals, vertices on a linear
+ *                          field's iso-plane, single-cell octahedron, per-field and
+ *                          channel-shared edge maps (tests/test_mesh_oracle.py).
  *   full step              pinned through the above plus the f32-vs-f64 self-consistency.
  */
 #ifndef LENIA_ORACLE_H
