@@ -188,6 +188,17 @@ lh_status lh_mesh(void *engine, int world, int view, float iso, float cell_size,
                   const int *tri_table, const uint32_t *colors, lh_vertex *verts, int64_t max_verts,
                   uint32_t *indices, int64_t max_indices, int64_t *n_verts, int64_t *n_indices);
 
+/* The same mesh built from the fields held in snapshot slot `slot` (lh_snapshot) --
+ * the mesh threads' RBB / U_SUM_RB / H_RB copies (main.cpp:212-244) -- on the
+ * read-back stream, so the steps enqueued after the snapshot keep running meanwhile.
+ * The snapshot must hold the view's fields (A; U_SUM; H).  Returns when the mesh is in
+ * the host buffers.  Calls of lh_mesh and lh_mesh_snapshot on one engine share its mesh
+ * buffers: make them from one host thread at a time. */
+lh_status lh_mesh_snapshot(void *engine, int slot, int world, int view, float iso, float cell_size,
+                           int face_only, const int *tri_table, const uint32_t *colors, lh_vertex *verts,
+                           int64_t max_verts, uint32_t *indices, int64_t max_indices, int64_t *n_verts,
+                           int64_t *n_indices);
+
 /* ---- checkpoints ----------------------------------------------------------------- */
 /* One world's complete state as a flat, versioned byte image: the intent of the
  * reference's disabled save_config / load_config (config.cpp:75-261: the simulation

@@ -1111,10 +1111,12 @@ lh_status lh_mc_table(int *tri) {
     return LH_OK;
 }
 
-lh_status lh_mesh(void *p, int world, int view, float iso, float cell_size, int face_only, const int *tri_table,
-                  const uint32_t *colors, lh_vertex *verts, int64_t max_verts, uint32_t *indices, int64_t max_indices,
-                  int64_t *n_verts, int64_t *n_indices) {
-    Engine *e = as_engine(p);
+// Checks shared by lh_mesh and lh_mesh_snapshot; fills the job's shape and parameters
+// and picks the table (tri_table, or the derived one in *table).
+static lh_status mesh_prepare(Engine *e, int world, int view, float iso, float cell_size, int face_only,
+                              const int *&tri_table, int *table, lh_vertex *verts, int64_t max_verts,
+                              uint32_t *indices, int64_t max_indices, int64_t *n_verts, int64_t *n_indices,
+                              MeshJob &job) {
     if (!e || !n_verts || !n_indices) return fail(LH_EINVAL, "null argument");
     if (max_verts < 0 || max_indices < 0) return fail(LH_EINVAL, "negative capacity");
     if ((max_verts > 0 && !verts) || (max_indices > 0 && !indices)) return fail(LH_EINVAL, "null output buffer");
@@ -1122,9 +1124,8 @@ lh_status lh_mesh(void *p, int world, int view, float iso, float cell_size, int 
     if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
     if (view < 1 || view > 3) return fail(LH_EINVAL, "view must be 1 (A), 2 (A, U_SUM) or 3 (A, U_SUM, H)");
     if (!std::isfinite(iso) || !std::isfinite(cell_size)) return fail(LH_EINVAL, "iso and cell_size must be finite");
-    HostWorld &hw = e->worlds[world];
+    const HostWorld &hw = e->worlds[world];
     if (view >= 2 && !hw.dev.keep_usum) return fail(LH_ESTATE, "views 2 and 3 need U_SUM: set LH_OPT_KEEP_USUM");
-    int table[256 * 16];
     if (!tri_table) {
         lh_mc_derive(table);
         tri_table = table;
@@ -1134,9 +1135,6 @@ lh_status lh_mesh(void *p, int world, int view, float iso, float cell_size, int 
     const int64_t N = (int64_t)hw.shape.W * hw.shape.H * hw.shape.D;
     if (N * hw.shape.C * view * 15 >= ((int64_t)1 << 32))
         return fail(LH_EUNSUPPORTED, "mesh of this world exceeds 32-bit vertex and index counts");
-    MeshJob job;
-    const int fields[3] = {LH_F_A, LH_F_USUM, LH_F_H};
-    for (int f = 0; f < 3; f++) job.field[f] = f < view ? e->pool + field_off(*e, hw, fields[f]) : nullptr;
     job.W = hw.shape.W;
     job.H = hw.shape.H;
     job.D = hw.shape.D;
@@ -1145,12 +1143,63 @@ lh_status lh_mesh(void *p, int world, int view, float iso, float cell_size, int 
     job.face_only = face_only ? 1 : 0;
     job.iso = iso;
     job.cell = cell_size;
-    HIPCHK(hipSetDevice(e->device));
-    HIPCHK(lh_mesh_run(job, tri_table, colors, e->mesh, e->stream, verts, max_verts, indices, max_indices, n_verts,
+    return LH_OK;
+}
+
+static lh_status mesh_finish(Engine *e, const MeshJob &job, const int *tri_table, const uint32_t *colors,
+                             hipStream_t st, lh_vertex *verts, int64_t max_verts, uint32_t *indices,
+                             int64_t max_indices, int64_t *n_verts, int64_t *n_indices) {
+    HIPCHK(lh_mesh_run(job, tri_table, colors, e->mesh, st, verts, max_verts, indices, max_indices, n_verts,
                        n_indices));
     if (verts && (*n_verts > max_verts || *n_indices > max_indices))
         return fail(LH_EINVAL, "mesh does not fit max_verts / max_indices (counts written)");
     return LH_OK;
+}
+
+lh_status lh_mesh(void *p, int world, int view, float iso, float cell_size, int face_only, const int *tri_table,
+                  const uint32_t *colors, lh_vertex *verts, int64_t max_verts, uint32_t *indices, int64_t max_indices,
+                  int64_t *n_verts, int64_t *n_indices) {
+    Engine *e = as_engine(p);
+    int table[256 * 16];
+    MeshJob job;
+    const lh_status st = mesh_prepare(e, world, view, iso, cell_size, face_only, tri_table, table, verts, max_verts,
+                                      indices, max_indices, n_verts, n_indices, job);
+    if (st != LH_OK) return st;
+    const HostWorld &hw = e->worlds[world];
+    const int fields[3] = {LH_F_A, LH_F_USUM, LH_F_H};
+    for (int f = 0; f < 3; f++) job.field[f] = f < view ? e->pool + field_off(*e, hw, fields[f]) : nullptr;
+    HIPCHK(hipSetDevice(e->device));
+    return mesh_finish(e, job, tri_table, colors, e->stream, verts, max_verts, indices, max_indices, n_verts,
+                       n_indices);
+}
+
+lh_status lh_mesh_snapshot(void *p, int slot, int world, int view, float iso, float cell_size, int face_only,
+                           const int *tri_table, const uint32_t *colors, lh_vertex *verts, int64_t max_verts,
+                           uint32_t *indices, int64_t max_indices, int64_t *n_verts, int64_t *n_indices) {
+    Engine *e = as_engine(p);
+    int table[256 * 16];
+    MeshJob job;
+    const lh_status st = mesh_prepare(e, world, view, iso, cell_size, face_only, tri_table, table, verts, max_verts,
+                                      indices, max_indices, n_verts, n_indices, job);
+    if (st != LH_OK) return st;
+    if (slot < 0 || slot > 1) return fail(LH_EINVAL, "slot must be 0 or 1");
+    const Snapshot &sn = e->snap[slot];
+    if (!sn.taken) return fail(LH_ESTATE, "no snapshot in this slot");
+    if (world < sn.first || world >= sn.first + sn.n) return fail(LH_EINVAL, "world not in this snapshot");
+    const int need[3] = {LH_F_A, LH_F_USUM, LH_F_H};
+    for (int f = 0; f < view; f++) {
+        int b = -1;
+        for (int q = 0; q < 3; q++)
+            if (SNAP_FIELD[q] == need[f]) b = q;
+        if (!(sn.fields & (1 << b))) return fail(LH_EINVAL, "a field of this view is not in the snapshot");
+        job.field[f] = sn.d + sn.off[(size_t)(world - sn.first) * 3 + b];
+    }
+    for (int f = view; f < 3; f++) job.field[f] = nullptr;
+    HIPCHK(hipSetDevice(e->device));
+    if (!e->rb_stream) HIPCHK(hipStreamCreateWithFlags(&e->rb_stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamWaitEvent(e->rb_stream, sn.done, 0));
+    return mesh_finish(e, job, tri_table, colors, e->rb_stream, verts, max_verts, indices, max_indices, n_verts,
+                       n_indices);
 }
 
 int lh_world_count(void *p) {

@@ -188,10 +188,13 @@ class Engine:
                                    out.ctypes.data_as(C.POINTER(C.c_uint8))), "lh_mc_case")
         return out.reshape(D, H, W)
 
-    def mesh(self, wid, view=1, iso=0.5, cell_size=3.5, face_only=True, tri_table=None, colors=None):
+    def mesh(self, wid, view=1, iso=0.5, cell_size=3.5, face_only=True, tri_table=None, colors=None,
+             snapshot_slot=None):
         """Marching-cubes mesh of world wid for view 1 (A), 2 (A, U_SUM) or 3 (A, U_SUM, H), as
         the renderer's mesh threads build it (mesh.cpp:34, 434, 1115): returns (vertices as
-        VERTEX_DTYPE records, uint32 indices).  tri_table: 256x16 ints, default mc_table()."""
+        VERTEX_DTYPE records, uint32 indices).  tri_table: 256x16 ints, default mc_table().
+        snapshot_slot: mesh the fields of that snapshot (lh_mesh_snapshot) instead of the
+        current state, while later steps keep running."""
         tri = None
         if tri_table is not None:
             tri = np.ascontiguousarray(tri_table, dtype=np.int32).reshape(256, 16)
@@ -204,6 +207,10 @@ class Engine:
         cp = col.ctypes.data_as(C.POINTER(C.c_uint32)) if col is not None else None
         nv, ni = C.c_int64(), C.c_int64()
         args = (self._h, wid, int(view), float(iso), float(cell_size), int(face_only), tp, cp)
+        fn, what = self._L.lh_mesh, "lh_mesh"
+        if snapshot_slot is not None:
+            args = (self._h, int(snapshot_slot)) + args[1:]
+            fn, what = self._L.lh_mesh_snapshot, "lh_mesh_snapshot"
         # one pass when the buffers of the last mesh of this size class are large enough,
         # else a second pass into buffers of the counts the first one reported
         key = (wid, int(view))
@@ -211,14 +218,14 @@ class Engine:
         for _ in range(2):
             verts = np.zeros(cap_v, dtype=VERTEX_DTYPE)
             idx = np.zeros(cap_i, dtype=np.uint32)
-            rc = self._L.lh_mesh(*args, verts.ctypes.data_as(C.c_void_p) if cap_v else None, cap_v,
+            rc = fn(*args, verts.ctypes.data_as(C.c_void_p) if cap_v else None, cap_v,
                                  idx.ctypes.data_as(C.POINTER(C.c_uint32)) if cap_i else None, cap_i,
                                  C.byref(nv), C.byref(ni))
             fits = nv.value <= cap_v and ni.value <= cap_i
             if rc == L.OK and fits and (cap_v or not nv.value):
                 break
             if fits:
-                L.check(rc, "lh_mesh")
+                L.check(rc, what)
             cap_v, cap_i = max(cap_v, int(nv.value * 1.25) + 64), max(cap_i, int(ni.value * 1.25) + 192)
         if not hasattr(self, "_mesh_caps"):
             self._mesh_caps = {}

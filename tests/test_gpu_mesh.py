@@ -130,3 +130,27 @@ def test_mesh_errors_and_size_query():
                      idx.ctypes.data_as(C_.POINTER(C_.c_uint32)), len(i), C_.byref(nv), C_.byref(ni))
     assert rc == L.EINVAL and nv.value == len(v)
     eng.close()
+
+
+def test_mesh_of_a_snapshot_while_steps_run():
+    """lh_mesh_snapshot meshes the fields a snapshot took (the mesh threads' RBB, U_SUM_RB,
+    H_RB) while 30 later steps are in flight: bit-identical to the checker on those fields,
+    and to lh_mesh of a second engine stepped to the same point."""
+    from lenia_amd.engine import Engine, mc_table
+    from oracle import oracle as O
+    shape, C = (32, 32, 32), 3
+    eng, wid = stepped(shape, C, seed=55, steps=3)
+    eng.snapshot(0, 0, 1, ("A", "USUM", "H"))
+    eng.step(30)
+    gpu = eng.mesh(wid, view=3, colors=COLORS[:C], snapshot_slot=0)
+    snap = [eng.snapshot_read(0, wid, n).reshape(C, 32, 32, 32) for n in ("A", "USUM", "H")]
+    same(gpu, O.mesh(snap, mc_table(), colors=COLORS[:C]))
+    ref_eng, ref_wid = stepped(shape, C, seed=55, steps=3)
+    same(gpu, ref_eng.mesh(ref_wid, view=3, colors=COLORS[:C]))
+    ref_eng.close()
+    live = eng.mesh(wid, view=3)  # the state 30 steps later meshes differently
+    assert not (len(live[1]) == len(gpu[1]) and np.array_equal(live[1], gpu[1]))
+    eng.snapshot(1, 0, 1, ("A",))
+    with pytest.raises(L.LeniaHipError, match="not in the snapshot"):
+        eng.mesh(wid, view=2, snapshot_slot=1)
+    eng.close()
