@@ -7,16 +7,18 @@
 // order of "units" u = ((channel * cubes + cube) * F + field): a vertex belongs to the
 // unit and triangle slot that first names its edge, vertices are numbered in that order,
 // the index buffer runs in unit order, and a vertex's normal is the sum of the face
-// normals of the triangles naming it, in unit order.  Here every rule is data-parallel
-// (one thread per channel and cube walks that cube's F units, so that neighbouring
-// threads read neighbouring cells):
-//   count   case index, triangle count, and an atomicMin of the 64-bit key
-//           u * 16 + slot on each named edge (the first owner wins);
-//   own     the slots whose key won;
+// normals of the triangles naming it, in unit order.  Here every rule is data-parallel:
+//   count   one thread per channel and cube walks that cube's F units (neighbouring
+//           threads read neighbouring cells): case index, triangle count, an atomicMin
+//           of the 64-bit key u * 16 + slot on each named edge (the first owner wins),
+//           and the unit appended to a list of units with triangles (any order; one
+//           global atomic per workgroup);
+//   own     per listed unit, the slots whose key won;
 //   scan    exclusive prefix sums of index and vertex counts over the units;
-//   emit    owned vertices at their ranks (position interpolated in the owning cube's
-//           corner order, as the reference stores it), edge -> vertex map;
-//   index   the index buffer and each triangle's face normal at its serial position;
+//   emit    per listed unit, owned vertices at their ranks (interpolated in the owning cube's
+//           corner order, as the reference stores it), edge -> vertex map, and the unit
+//           of each triangle;
+//   index   one thread per triangle: its indices and face normal at its serial position;
 //   normal  per vertex: a gather over the (channel, adjacent cube, triangle) sequence
 //           naming it, summed in the reference's order, so the fp32 sums match.
 // All interpolation and cross products are written as the reference's expressions
@@ -76,6 +78,8 @@ struct MeshDev {
     uint32_t *vmap;        // [F * E3] edge -> vertex
     uint8_t *ucase;        // [U]
     uint32_t *cnt_i, *cnt_v;  // [U] counts, then exclusive offsets
+    uint32_t *active;      // [U] units with triangles, in no particular order
+    unsigned int *nact;    // their number
     long long U, Nc, E, E3;
 };
 
@@ -84,6 +88,7 @@ struct MeshOut {
     uint32_t *idx;    // [ni]
     float *fnorm;     // [ni / 3][3] face normal of each triangle
     uint32_t *vedge;  // [nv] edge slot (field * E3 + edge) of each vertex
+    uint32_t *tunit;  // [ni / 3] unit of each triangle
 };
 
 __device__ __forceinline__ float corner_value(const float *f, const MeshJob &j, int cx, int cy, int cz) {
@@ -155,32 +160,63 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *p, long lo
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
 }
 
+__device__ __forceinline__ void unit_cube(const MeshDev &m, long long u, int &c, int &f, int &x, int &y, int &z) {
+    f = (int)(u % m.j.F);
+    const long long t = u / m.j.F;
+    const long long cube = t % m.Nc;
+    c = (int)(t / m.Nc);
+    x = (int)(cube % m.j.W);
+    y = (int)((cube / m.j.W) % m.j.H);
+    z = (int)(cube / ((long long)m.j.W * m.j.H));
+}
+
 __global__ __launch_bounds__(256) void k_mesh_count(MeshDev m) {
+    __shared__ unsigned int wg_n, wg_base;
+    if (threadIdx.x == 0) wg_n = 0u;
+    __syncthreads();
     long long t;
     int c, x, y, z;
-    if (!cube_coords(m, t, c, x, y, z)) return;
-    for (int f = 0; f < m.j.F; f++) {
-        const long long u = t * m.j.F + f;
-        float v[8];
-        load_corners(m, c, f, x, y, z, v);
-        const int cs = case_of(v, m.j.iso);
-        m.ucase[u] = (uint8_t)cs;
-        const int ns = 3 * m.ntri[cs];
-        m.cnt_i[u] = (uint32_t)ns;
-        const int8_t *row = m.tri + 16 * cs;
-        for (int s = 0; s < ns; s++) {
-            const long long g = f * m.E3 + edge_slot(m, x, y, z, row[s]);
-            atomicMin(m.first + g, (unsigned long long)u * 16ull + (unsigned long long)s);
+    const bool valid = cube_coords(m, t, c, x, y, z);
+    int act = 0;  // bit f: unit t * F + f has triangles
+    if (valid) {
+        for (int f = 0; f < m.j.F; f++) {
+            const long long u = t * m.j.F + f;
+            float v[8];
+            load_corners(m, c, f, x, y, z, v);
+            const int cs = case_of(v, m.j.iso);
+            m.ucase[u] = (uint8_t)cs;
+            const int ns = 3 * m.ntri[cs];
+            m.cnt_i[u] = (uint32_t)ns;
+            const int8_t *row = m.tri + 16 * cs;
+            for (int s = 0; s < ns; s++) {
+                const long long g = f * m.E3 + edge_slot(m, x, y, z, row[s]);
+                atomicMin(m.first + g, (unsigned long long)u * 16ull + (unsigned long long)s);
+            }
+            if (ns) act |= 1 << f;
         }
+    }
+    // append the units with triangles to the active list: slots within the workgroup
+    // from an LDS counter, one global atomic per workgroup
+    const unsigned int na = (unsigned int)__popc(act);
+    unsigned int off = 0;
+    if (na) off = atomicAdd(&wg_n, na);
+    __syncthreads();
+    if (threadIdx.x == 0) wg_base = wg_n ? atomicAdd(m.nact, wg_n) : 0u;
+    __syncthreads();
+    if (na) {
+        unsigned int p = wg_base + off;
+        for (int f = 0; f < m.j.F; f++)
+            if ((act >> f) & 1) m.active[p++] = (uint32_t)(t * m.j.F + f);
     }
 }
 
+// Owned slots of each active unit (cnt_v of the others stays 0).
 __global__ __launch_bounds__(256) void k_mesh_own(MeshDev m) {
-    long long t;
-    int c, x, y, z;
-    if (!cube_coords(m, t, c, x, y, z)) return;
-    for (int f = 0; f < m.j.F; f++) {
-        const long long u = t * m.j.F + f;
+    const unsigned int n = *m.nact;
+    for (unsigned int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const long long u = m.active[i];
+        int c, f, x, y, z;
+        unit_cube(m, u, c, f, x, y, z);
         const int cs = m.ucase[u];
         const int ns = 3 * m.ntri[cs];
         const int8_t *row = m.tri + 16 * cs;
@@ -300,69 +336,65 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(uint32_t *a, uint32_t *b,
     }
 }
 
-__global__ __launch_bounds__(256) void k_mesh_emit(MeshDev m, MeshOut o) {
-    long long t;
-    int c, x, y, z;
-    if (!cube_coords(m, t, c, x, y, z)) return;
-    for (int f = 0; f < m.j.F; f++) {
-        const long long u = t * m.j.F + f;
-        const int cs = m.ucase[u];
-        const int ns = 3 * m.ntri[cs];
-        if (!ns) continue;
-        float v[8];
-        load_corners(m, c, f, x, y, z, v);
-        const int8_t *row = m.tri + 16 * cs;
-        uint32_t vid = m.cnt_v[u];
-        for (int s = 0; s < ns; s++) {
-            const int e = row[s];
-            const long long g = f * m.E3 + edge_slot(m, x, y, z, e);
-            if (m.first[g] != (unsigned long long)u * 16ull + (unsigned long long)s) continue;
-            float px, py, pz;
-            edge_point(m, v, x, y, z, e, px, py, pz);
-            float *out = o.verts + 7ll * vid;
-            out[0] = px;
-            out[1] = py;
-            out[2] = pz;
-            out[6] = __uint_as_float(m.colors[c]);
-            m.vmap[g] = vid;
-            o.vedge[vid] = (uint32_t)g;
-            vid++;
-        }
+__global__ __launch_bounds__(256) void k_mesh_emit(MeshDev m, MeshOut o, unsigned int nact) {
+    const unsigned int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nact) return;
+    const long long u = m.active[i];
+    int c, f, x, y, z;
+    unit_cube(m, u, c, f, x, y, z);
+    const int cs = m.ucase[u];
+    const int ns = 3 * m.ntri[cs];
+    float v[8];
+    load_corners(m, c, f, x, y, z, v);
+    const int8_t *row = m.tri + 16 * cs;
+    uint32_t *tu = o.tunit + m.cnt_i[u] / 3;
+    for (int tri = 0; tri < ns / 3; tri++) tu[tri] = (uint32_t)u;
+    uint32_t vid = m.cnt_v[u];
+    for (int s = 0; s < ns; s++) {
+        const int e = row[s];
+        const long long g = f * m.E3 + edge_slot(m, x, y, z, e);
+        if (m.first[g] != (unsigned long long)u * 16ull + (unsigned long long)s) continue;
+        float px, py, pz;
+        edge_point(m, v, x, y, z, e, px, py, pz);
+        float *out = o.verts + 7ll * vid;
+        out[0] = px;
+        out[1] = py;
+        out[2] = pz;
+        out[6] = __uint_as_float(m.colors[c]);
+        m.vmap[g] = vid;
+        o.vedge[vid] = (uint32_t)g;
+        vid++;
     }
 }
 
-// The index buffer, and the face normal of every triangle (mesh.cpp:229-300) from the
-// cube's own interpolated corners, at the triangle's serial position.
-__global__ __launch_bounds__(256) void k_mesh_index(MeshDev m, MeshOut o) {
-    long long t;
-    int c, x, y, z;
-    if (!cube_coords(m, t, c, x, y, z)) return;
-    for (int f = 0; f < m.j.F; f++) {
-        const long long u = t * m.j.F + f;
-        const int cs = m.ucase[u];
-        const int nt = m.ntri[cs];
-        if (!nt) continue;
-        float v[8];
-        load_corners(m, c, f, x, y, z, v);
-        const int8_t *row = m.tri + 16 * cs;
-        const uint32_t base = m.cnt_i[u];
-        for (int tri = 0; tri < nt; tri++) {
-            const int e0 = row[3 * tri], e1 = row[3 * tri + 1], e2 = row[3 * tri + 2];
-            o.idx[base + 3 * tri] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e0)];
-            o.idx[base + 3 * tri + 1] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e1)];
-            o.idx[base + 3 * tri + 2] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e2)];
-            float ax, ay, az, bx, by, bz, qx, qy, qz;
-            edge_point(m, v, x, y, z, e0, ax, ay, az);
-            edge_point(m, v, x, y, z, e1, bx, by, bz);
-            edge_point(m, v, x, y, z, e2, qx, qy, qz);
-            const float ux = bx - ax, uy = by - ay, uz = bz - az;
-            const float vx = qx - ax, vy = qy - ay, vz = qz - az;
-            float *fn = o.fnorm + base + 3 * tri;
-            fn[0] = uy * vz - uz * vy;
-            fn[1] = uz * vx - ux * vz;
-            fn[2] = ux * vy - uy * vx;
-        }
-    }
+// The index buffer, and the face normal of every triangle (mesh.cpp:229-300) from its
+// cube's own interpolated corners: one thread per triangle, at its serial position.
+__global__ __launch_bounds__(256) void k_mesh_index(MeshDev m, MeshOut o, long long ntri) {
+    const long long tau = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (tau >= ntri) return;
+    const long long u = o.tunit[tau];
+    const uint32_t base = m.cnt_i[u];
+    const int tri = (int)(tau - base / 3);
+    int c, f, x, y, z;
+    unit_cube(m, u, c, f, x, y, z);
+    const int8_t *row = m.tri + 16 * m.ucase[u];
+    const int e0 = row[3 * tri], e1 = row[3 * tri + 1], e2 = row[3 * tri + 2];
+    uint32_t *ix = o.idx + 3 * tau;
+    ix[0] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e0)];
+    ix[1] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e1)];
+    ix[2] = m.vmap[f * m.E3 + edge_slot(m, x, y, z, e2)];
+    float v[8];
+    load_corners(m, c, f, x, y, z, v);
+    float ax, ay, az, bx, by, bz, qx, qy, qz;
+    edge_point(m, v, x, y, z, e0, ax, ay, az);
+    edge_point(m, v, x, y, z, e1, bx, by, bz);
+    edge_point(m, v, x, y, z, e2, qx, qy, qz);
+    const float ux = bx - ax, uy = by - ay, uz = bz - az;
+    const float vx = qx - ax, vy = qy - ay, vz = qz - az;
+    float *fn = o.fnorm + 3 * tau;
+    fn[0] = uy * vz - uz * vy;
+    fn[1] = uz * vx - ux * vz;
+    fn[2] = ux * vy - uy * vx;
 }
 
 // Normal of vertex i: the face normals of every triangle naming it, summed over
@@ -592,12 +624,14 @@ hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *color
     off = up256(off + (size_t)m.U * sizeof(uint32_t));
     const size_t o_cv = off;
     off = up256(off + (size_t)m.U * sizeof(uint32_t));
+    const size_t o_act = off;
+    off = up256(off + (size_t)m.U * sizeof(uint32_t));
     const size_t o_ta = off;
     off = up256(off + (size_t)ntiles * sizeof(uint32_t));
     const size_t o_tb = off;
     off = up256(off + (size_t)ntiles * sizeof(uint32_t));
     const size_t o_tot = off;
-    off = up256(off + 2 * sizeof(uint32_t));
+    off = up256(off + 3 * sizeof(uint32_t));  // index and vertex totals, active units
     hipError_t e = grow(&b.work, &b.work_cap, off);
     if (e != hipSuccess) return e;
     char *w = (char *)b.work;
@@ -621,34 +655,41 @@ hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *color
     m.ucase = (uint8_t *)(w + o_case);
     m.cnt_i = (uint32_t *)(w + o_ci);
     m.cnt_v = (uint32_t *)(w + o_cv);
+    m.active = (uint32_t *)(w + o_act);
     uint32_t *ta = (uint32_t *)(w + o_ta), *tb = (uint32_t *)(w + o_tb), *tot = (uint32_t *)(w + o_tot);
+    m.nact = tot + 2;
     if ((e = hipMemcpyAsync(w + o_tri, htri, sizeof(htri), hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(w + o_ntri, hn, sizeof(hn), hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(w + o_col, hcol, sizeof(hcol), hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     const long long nfirst = job.F * m.E3;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)std::min<long long>(grid_for(nfirst), 4096)), dim3(256), 0, st,
                        m.first, nfirst, ~0ull);
+    if ((e = hipMemsetAsync(m.cnt_v, 0, (size_t)m.U * sizeof(uint32_t), st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(m.nact, 0, sizeof(uint32_t), st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_mesh_count, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m);
-    hipLaunchKernelGGL(k_mesh_own, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_mesh_own, dim3((unsigned)std::min<long long>(grid_for(m.U), 2048)), dim3(256), 0, st, m);
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.cnt_i, m.cnt_v, m.U, ta, tb);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, ta, tb, ntiles, tot);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.cnt_i, m.cnt_v, m.U, ta, tb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    uint32_t htot[2];
+    uint32_t htot[3];
     if ((e = hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     *ni = htot[0];
     *nv = htot[1];
     if (!verts || !idx || *nv > max_v || *ni > max_i || *nv == 0) return hipSuccess;
+    const long long ntri = *ni / 3;
     const size_t vb = up256((size_t)*nv * 28), ib = up256((size_t)*ni * 4), eb = up256((size_t)*nv * 4);
-    if ((e = grow(&b.out, &b.out_cap, vb + 2 * ib + eb)) != hipSuccess) return e;
+    const size_t ub = up256((size_t)ntri * 4);
+    if ((e = grow(&b.out, &b.out_cap, vb + 2 * ib + eb + ub)) != hipSuccess) return e;
     MeshOut o;
     o.verts = (float *)b.out;
     o.idx = (uint32_t *)((char *)b.out + vb);
     o.fnorm = (float *)((char *)b.out + vb + ib);
     o.vedge = (uint32_t *)((char *)b.out + vb + 2 * ib);
-    hipLaunchKernelGGL(k_mesh_emit, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m, o);
-    hipLaunchKernelGGL(k_mesh_index, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m, o);
+    o.tunit = (uint32_t *)((char *)b.out + vb + 2 * ib + eb);
+    hipLaunchKernelGGL(k_mesh_emit, dim3(grid_for(htot[2])), dim3(256), 0, st, m, o, htot[2]);
+    hipLaunchKernelGGL(k_mesh_index, dim3(grid_for(ntri)), dim3(256), 0, st, m, o, ntri);
     hipLaunchKernelGGL(k_mesh_normal, dim3(grid_for(*nv)), dim3(256), 0, st, m, o, (long long)*nv);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(verts, o.verts, (size_t)*nv * 28, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
