@@ -98,6 +98,10 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
     }
 }
 
+// Below this many columns in a batch (256 CUs x 256 lanes), spec_z runs one thread per
+// column and kernel rather than one per column.
+constexpr long long SPEC_SPLIT_COLUMNS = 65536;
+
 // ------------------------------------------------------------------------------------
 // spec_z: per (kx, ky) column.  mode 0: for each channel used as a kernel input, z-FFT,
 // then for each kernel reading it: x real kernel octant, inverse z-FFT -> G_k.
@@ -145,6 +149,42 @@ __global__ __launch_bounds__(128) void k_spec_z(PhaseArgs a, int mode) {
             for (int z = 0; z < D; z++) dst[(long long)z * H * Wh] = y[z];
         }
     }
+}
+
+// spec_z (mode 0) for batches too small to fill the GPU with one thread per column:
+// one thread per (column, kernel).  Each thread repeats the forward z-FFT of its
+// kernel's source column -- the same arithmetic as k_spec_z, so the same bits -- and
+// its chain shrinks from 1 + K transforms to 2.
+template <int W, int H, int D>
+__global__ __launch_bounds__(128) void k_spec_z_split(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    constexpr int Wh = S::Wh, Hh = S::Hh, COLS = Wh * H;
+    constexpr int BPW = (COLS + 127) / 128;
+    const int k = blockIdx.y;
+    const int wi = blockIdx.x / BPW;
+    const int col = (blockIdx.x % BPW) * 128 + threadIdx.x;
+    if (col >= COLS) return;
+    const DevWorld &w = a.worlds[a.ids[wi]];
+    if (k >= w.K) return;
+    const int kx = col % Wh, ky = col / Wh;
+    const int kyo = ky <= H / 2 ? ky : H - ky;
+    const float2 *src = reinterpret_cast<const float2 *>(a.pool + w.Fsp) + (long long)w.kern[k].c0 * S::Nh +
+                        ky * Wh + kx;
+    float2 t[D];
+#pragma unroll
+    for (int z = 0; z < D; z++) t[z] = src[(long long)z * H * Wh];
+    fft_line<D, false>(t, a.twD);
+    const float *oct = a.pool + w.KH + (long long)k * S::Noct;
+#pragma unroll
+    for (int z = 0; z < D; z++) {
+        const int zo = z <= D / 2 ? z : D - z;
+        const float kv = oct[(zo * Hh + kyo) * Wh + kx];
+        t[z] = make_float2(t[z].x * kv, t[z].y * kv);
+    }
+    fft_line<D, true>(t, a.twD);
+    float2 *dst = reinterpret_cast<float2 *>(a.pool + w.Gsp) + (long long)k * S::Nh + ky * Wh + kx;
+#pragma unroll
+    for (int z = 0; z < D; z++) dst[(long long)z * H * Wh] = t[z];
 }
 
 // kspec_z: forward z-FFT of staged kernel spectra; keeps the real part of the octant.
@@ -387,8 +427,12 @@ struct Launch {
         hipLaunchKernelGGL((k_fwd_xy<W, H, D>), dim3(grid), dim3(128), 0, (hipStream_t)st, a, arg);
     }
     static void spec_z(const PhaseArgs &a, int mode, void *st) {
-        constexpr int BPW = ((W / 2 + 1) * H + 127) / 128;
-        hipLaunchKernelGGL((k_spec_z<W, H, D>), dim3(a.nworlds * BPW), dim3(128), 0, (hipStream_t)st, a, mode);
+        constexpr int COLS = (W / 2 + 1) * H, BPW = (COLS + 127) / 128;
+        if (mode == 0 && (long long)a.nworlds * COLS < SPEC_SPLIT_COLUMNS)
+            hipLaunchKernelGGL((k_spec_z_split<W, H, D>), dim3(a.nworlds * BPW, LHD_MAX_K), dim3(128), 0,
+                               (hipStream_t)st, a, 0);
+        else
+            hipLaunchKernelGGL((k_spec_z<W, H, D>), dim3(a.nworlds * BPW), dim3(128), 0, (hipStream_t)st, a, mode);
     }
     static void kspec_z(const PhaseArgs &a, int, void *st) {
         constexpr int BPW = ((W / 2 + 1) * H + 127) / 128;

@@ -187,6 +187,34 @@ def test_batched_mixed_shapes_match_single_worlds():
         assert bad == 0, (i, w["shape"], w["cfg"]["mode"], bad, err)
 
 
+def test_large_batch_one_thread_per_column_spectral_path():
+    """128 worlds of 32^3 (69,632 spectral columns) take spec_z's one-thread-per-column
+    schedule; smaller batches take the per-(column, kernel) one.  Spot-checked worlds of
+    the large batch equal the CPU checker, and equal themselves run alone."""
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    ws = [WL.plain32(seed=700 + i, asymptotic=ASYM) for i in range(128)]
+    eng = Engine()
+    for w in ws:
+        eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    for i, w in enumerate(ws):
+        eng.set(i, "A", w["A"])
+    eng.step(3)
+    eng.synchronize()
+    batch = {i: eng.get(i, "A") for i in (0, 63, 127)}
+    eng.close()
+    for i, a in batch.items():
+        w = ws[i]
+        ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 32)
+        ow.set(O.F_A, w["A"])
+        ow.step(3)
+        bad, err = mismatch(a, ow.get(O.F_A))
+        assert bad == 0, (i, bad, err)
+        alone = run_pair(w, 3)["A"][0]
+        assert np.array_equal(alone.view(np.uint32), a.view(np.uint32)), i
+
+
 def test_set_field_midrun_and_state_roundtrip():
     from lenia_amd.engine import Engine
     from oracle import oracle as O
