@@ -31,11 +31,12 @@ struct FwdCfg {
     static constexpr int THREADS = 128;
 };
 
-template <int W, int H, int D>
+// NP: z-planes per workgroup (FwdCfg::NP, halved for batches of few workgroups).
+template <int W, int H, int D, int NP>
 __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
     using S = Shape<W, H, D>;
     using Cfg = FwdCfg<W, H, D>;
-    constexpr int NP = Cfg::NP, Wh = S::Wh;
+    constexpr int Wh = S::Wh;
     __shared__ float2 sspec[NP * H * Wh];
     const int src = arg & 0xFF, nfmax = arg >> 8;
     const int groups = D / NP;
@@ -101,6 +102,10 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
 // Below this many columns in a batch (256 CUs x 256 lanes), spec_z runs one thread per
 // column and kernel rather than one per column.
 constexpr long long SPEC_SPLIT_COLUMNS = 65536;
+// Below this many workgroups, fwd_xy gives each workgroup half as many z-planes and
+// inv_growth splits each plane over two workgroups.
+constexpr int FWD_SPLIT_WORKGROUPS = 1024;
+constexpr int INV_SPLIT_WORKGROUPS = 1024;
 
 // ------------------------------------------------------------------------------------
 // spec_z: per (kx, ky) column.  mode 0: for each channel used as a kernel input, z-FFT,
@@ -247,16 +252,21 @@ struct InvCfg {
 };
 
 
-// CM: channels per world at most (3 or 4); KU: U_SUM is written.
-template <int W, int H, int D, int CM, bool KU>
+// CM: channels per world at most (3 or 4); KU: U_SUM is written; SPL: workgroups per
+// plane (2 for batches of few workgroups: each repeats the y transforms, then takes
+// half the rows through the x transforms and the growth).
+template <int W, int H, int D, int CM, bool KU, int SPL>
 __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
     using Cfg = InvCfg<W, H, D>;
     constexpr int Wh = S::Wh, PL = S::PL, KB = Cfg::KB, T = Cfg::THREADS;
-    constexpr int CPT = (PL + T - 1) / T;  // cells of the plane owned by each thread
-    constexpr int CU = KU ? CM : 1;        // U_SUM accumulators
+    static_assert(H % (2 * SPL) == 0, "row pairs split evenly");
+    constexpr int PLS = PL / SPL, HP = H / 2 / SPL;  // cells and row pairs of this workgroup
+    constexpr int CPT = (PLS + T - 1) / T;  // cells owned by each thread
+    constexpr int CU = KU ? CM : 1;         // U_SUM accumulators
     __shared__ __attribute__((aligned(16))) float2 sg[KB * Cfg::PLANE];
-    const int wi = blockIdx.x / D, z = blockIdx.x % D;
+    const int wi = blockIdx.x / (D * SPL), z = (blockIdx.x / SPL) % D, part = blockIdx.x % SPL;
+    const int c0 = part * PLS;  // first cell of this workgroup in the plane
     const DevWorld &w = a.worlds[a.ids[wi]];
     const int tid = threadIdx.x, C = w.C, K = w.K;
     const bool lut = (w.quirks & LHQ_GROWTH_LUT) != 0;
@@ -291,8 +301,8 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
             for (int y = 0; y < H; y++) p[y * Wh] = line[y];
         }
         __syncthreads();
-        for (int t = tid; t < nk * (H / 2); t += T) {
-            const int kk = t / (H / 2), y = 2 * (t % (H / 2));
+        for (int t = tid; t < nk * HP; t += T) {
+            const int kk = t / HP, y = 2 * (t % HP) + part * (H / SPL);
             const float2 *ra = sg + kk * Cfg::PLANE + y * Wh, *rb = ra + Wh;
             float2 line[W];
             line[0] = make_float2(ra[0].x, rb[0].x);
@@ -316,8 +326,8 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
             const bool asym = (w.asym_mask >> c1) & 1;
 #pragma unroll
             for (int j = 0; j < CPT; j++) {
-                const int i = tid + j * T;
-                if (i >= PL) break;
+                const int i = c0 + tid + j * T;
+                if (tid + j * T >= PLS) break;
                 const int y = i / W, x = i % W;
                 const float u = sgf[2 * (kk * Cfg::PLANE + y * Wh) + x] / (float)S::N;
                 float g;
@@ -342,8 +352,8 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     float *US = a.pool + w.USUM + (long long)z * PL;
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-        const int i = tid + j * T;
-        if (i >= PL) break;
+        const int i = c0 + tid + j * T;
+        if (tid + j * T >= PLS) break;
 #pragma unroll
         for (int c = 0; c < CM; c++) {
             if (c >= C) break;
@@ -423,8 +433,13 @@ template <int W, int H, int D>
 struct Launch {
     // arg = source | (max fields per world) << 8
     static void fwd_xy(const PhaseArgs &a, int arg, void *st) {
-        const int grid = a.nworlds * (arg >> 8) * (D / FwdCfg<W, H, D>::NP);
-        hipLaunchKernelGGL((k_fwd_xy<W, H, D>), dim3(grid), dim3(128), 0, (hipStream_t)st, a, arg);
+        constexpr int NP = FwdCfg<W, H, D>::NP, NP2 = NP > 1 ? NP / 2 : 1;
+        const int grid = a.nworlds * (arg >> 8) * (D / NP);
+        if (grid < FWD_SPLIT_WORKGROUPS && NP2 < NP)  // few workgroups: half the planes each, twice as many
+            hipLaunchKernelGGL((k_fwd_xy<W, H, D, NP2>), dim3(a.nworlds * (arg >> 8) * (D / NP2)), dim3(128), 0,
+                               (hipStream_t)st, a, arg);
+        else
+            hipLaunchKernelGGL((k_fwd_xy<W, H, D, NP>), dim3(grid), dim3(128), 0, (hipStream_t)st, a, arg);
     }
     static void spec_z(const PhaseArgs &a, int mode, void *st) {
         constexpr int COLS = (W / 2 + 1) * H, BPW = (COLS + 127) / 128;
@@ -442,10 +457,20 @@ struct Launch {
     static void inv_growth(const PhaseArgs &a, int arg, void *st) {
         const dim3 grid(a.nworlds * D);
         const bool ku = arg & 1, c3 = (arg >> 8) <= 3;
-        if (c3 && !ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, false>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
-        else if (c3) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, true>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
-        else if (!ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, false>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
-        else hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, true>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        if constexpr (H % 4 == 0) {
+            if (a.nworlds * D < INV_SPLIT_WORKGROUPS) {
+                const dim3 g2(a.nworlds * D * 2);
+                if (c3 && !ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, false, 2>), g2, dim3(256), 0, (hipStream_t)st, a, 0);
+                else if (c3) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, true, 2>), g2, dim3(256), 0, (hipStream_t)st, a, 0);
+                else if (!ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, false, 2>), g2, dim3(256), 0, (hipStream_t)st, a, 0);
+                else hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, true, 2>), g2, dim3(256), 0, (hipStream_t)st, a, 0);
+                return;
+            }
+        }
+        if (c3 && !ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, false, 1>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else if (c3) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, true, 1>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else if (!ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, false, 1>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
+        else hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, true, 1>), grid, dim3(256), 0, (hipStream_t)st, a, 0);
     }
     static void inv_blur(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_inv_blur<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
