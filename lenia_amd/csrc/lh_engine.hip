@@ -408,8 +408,8 @@ lh_status autotune(Engine &e) {
         PhaseArgs a = args_for(e, c, 0);
         const int D = c.D;
         std::vector<int> cands = {D};
-        for (int zs : {D / 2, D / 4, 8, 4, 2})
-            if (zs >= 2 && std::find(cands.begin(), cands.end(), zs) == cands.end()) cands.push_back(zs);
+        for (int zs : {D / 2, D / 4, 8, 4, 2, 1})
+            if (zs >= 1 && std::find(cands.begin(), cands.end(), zs) == cands.end()) cands.push_back(zs);
         const int reps = c.ids.size() >= 256 ? 2 : 5;
         float best = 1e30f;
         int pick_local = D;
