@@ -103,7 +103,8 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
 // column and kernel rather than one per column.
 constexpr long long SPEC_SPLIT_COLUMNS = 65536;
 // Below this many workgroups, fwd_xy gives each workgroup half as many z-planes and
-// inv_growth splits each plane over two workgroups.
+// inv_growth splits each plane over two workgroups (below a quarter of it: a quarter of
+// the planes, four workgroups per plane).
 constexpr int FWD_SPLIT_WORKGROUPS = 1024;
 constexpr int INV_SPLIT_WORKGROUPS = 1024;
 
@@ -253,8 +254,8 @@ struct InvCfg {
 
 
 // CM: channels per world at most (3 or 4); KU: U_SUM is written; SPL: workgroups per
-// plane (2 for batches of few workgroups: each repeats the y transforms, then takes
-// half the rows through the x transforms and the growth).
+// plane (2 or 4 for batches of few workgroups: each repeats the y transforms, then
+// takes its share of the rows through the x transforms and the growth).
 template <int W, int H, int D, int CM, bool KU, int SPL>
 __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     using S = Shape<W, H, D>;
@@ -433,9 +434,12 @@ template <int W, int H, int D>
 struct Launch {
     // arg = source | (max fields per world) << 8
     static void fwd_xy(const PhaseArgs &a, int arg, void *st) {
-        constexpr int NP = FwdCfg<W, H, D>::NP, NP2 = NP > 1 ? NP / 2 : 1;
+        constexpr int NP = FwdCfg<W, H, D>::NP, NP2 = NP > 1 ? NP / 2 : 1, NP4 = NP > 3 ? NP / 4 : NP2;
         const int grid = a.nworlds * (arg >> 8) * (D / NP);
-        if (grid < FWD_SPLIT_WORKGROUPS && NP2 < NP)  // few workgroups: half the planes each, twice as many
+        if (grid < FWD_SPLIT_WORKGROUPS / 4 && NP4 < NP2)  // very few workgroups: a quarter of the planes each
+            hipLaunchKernelGGL((k_fwd_xy<W, H, D, NP4>), dim3(a.nworlds * (arg >> 8) * (D / NP4)), dim3(128), 0,
+                               (hipStream_t)st, a, arg);
+        else if (grid < FWD_SPLIT_WORKGROUPS && NP2 < NP)  // few workgroups: half the planes each, twice as many
             hipLaunchKernelGGL((k_fwd_xy<W, H, D, NP2>), dim3(a.nworlds * (arg >> 8) * (D / NP2)), dim3(128), 0,
                                (hipStream_t)st, a, arg);
         else
@@ -457,6 +461,16 @@ struct Launch {
     static void inv_growth(const PhaseArgs &a, int arg, void *st) {
         const dim3 grid(a.nworlds * D);
         const bool ku = arg & 1, c3 = (arg >> 8) <= 3;
+        if constexpr (H % 8 == 0) {
+            if (a.nworlds * D < INV_SPLIT_WORKGROUPS / 4) {
+                const dim3 g4(a.nworlds * D * 4);
+                if (c3 && !ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, false, 4>), g4, dim3(256), 0, (hipStream_t)st, a, 0);
+                else if (c3) hipLaunchKernelGGL((k_inv_growth<W, H, D, 3, true, 4>), g4, dim3(256), 0, (hipStream_t)st, a, 0);
+                else if (!ku) hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, false, 4>), g4, dim3(256), 0, (hipStream_t)st, a, 0);
+                else hipLaunchKernelGGL((k_inv_growth<W, H, D, LHD_MAX_C, true, 4>), g4, dim3(256), 0, (hipStream_t)st, a, 0);
+                return;
+            }
+        }
         if constexpr (H % 4 == 0) {
             if (a.nworlds * D < INV_SPLIT_WORKGROUPS) {
                 const dim3 g2(a.nworlds * D * 2);
