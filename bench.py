@@ -6,6 +6,7 @@ next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
                   [--no-mixed] [--no-precision] [--no-proposal] [--no-checkpoint] [--no-sampler]
+                  [--no-mesh]
 
 A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
 per-world SRT or RT) and reports it under "mixed_batch"; a third steps one 128^3 world
@@ -14,7 +15,9 @@ with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
 the proposal network ("proposal": scoring 10^6 sampled parameter sets in one call,
 training throughput); "checkpoint" times save and restore of world checkpoint images;
 "random_worlds" times the generate_random-range sampler and the Gaussian seeding;
-"snapshot_readback" reads a snapshot back while later steps run.
+"snapshot_readback" reads a snapshot back while later steps run; "mesh" times the
+marching-cubes meshes of views 1 and 3 for a 32^3 and a 128^3 world beside the CPU
+restatement of the mesh thread (parity checked bit for bit).
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
