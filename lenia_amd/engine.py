@@ -216,8 +216,8 @@ class Engine:
         key = (wid, int(view))
         cap_v, cap_i = getattr(self, "_mesh_caps", {}).get(key, (0, 0))
         for _ in range(2):
-            verts = np.zeros(cap_v, dtype=VERTEX_DTYPE)
-            idx = np.zeros(cap_i, dtype=np.uint32)
+            verts = np.empty(cap_v, dtype=VERTEX_DTYPE)
+            idx = np.empty(cap_i, dtype=np.uint32)
             rc = fn(*args, verts.ctypes.data_as(C.c_void_p) if cap_v else None, cap_v,
                                  idx.ctypes.data_as(C.POINTER(C.c_uint32)) if cap_i else None, cap_i,
                                  C.byref(nv), C.byref(ni))
