@@ -77,7 +77,7 @@ struct MeshDev {
     unsigned long long *first;  // [F * E3] lowest key naming each edge
     uint32_t *vmap;        // [F * E3] edge -> vertex
     uint8_t *ucase;        // [U]
-    uint32_t *cnt_i, *cnt_v;  // [U] counts, then exclusive offsets
+    uint32_t *cnt_i, *cnt_v;  // [U] exclusive offsets of indices and vertices (cnt_v: counts first)
     uint32_t *active;      // [U] units with triangles, in no particular order
     unsigned int *nact;    // their number
     long long U, Nc, E, E3;
@@ -184,9 +184,8 @@ __global__ __launch_bounds__(256) void k_mesh_count(MeshDev m) {
             float v[8];
             load_corners(m, c, f, x, y, z, v);
             const int cs = case_of(v, m.j.iso);
-            m.ucase[u] = (uint8_t)cs;
+            m.ucase[u] = (uint8_t)cs;  // the scan reads index counts from it
             const int ns = 3 * m.ntri[cs];
-            m.cnt_i[u] = (uint32_t)ns;
             const int8_t *row = m.tri + 16 * cs;
             for (int s = 0; s < ns; s++) {
                 const long long g = f * m.E3 + edge_slot(m, x, y, z, row[s]);
@@ -229,36 +228,42 @@ __global__ __launch_bounds__(256) void k_mesh_own(MeshDev m) {
     }
 }
 
-// ---- exclusive prefix sums of two uint32 arrays at once ---------------------------
+// ---- exclusive prefix sums of the index and vertex counts at once ------------------
+// Index counts are read as 3 * ntri[case] from the case bytes; the offsets go to a.
 constexpr int SCAN_T = 256, SCAN_PER = 8, SCAN_TILE = SCAN_T * SCAN_PER;
+static_assert(SCAN_T == 256, "one thread per case fills the index-count table");
 
+template <int NT = SCAN_T>
 __device__ __forceinline__ uint2 block_exclusive(uint2 x, uint2 *lds, uint2 &total) {
     const int t = threadIdx.x;
     lds[t] = x;
     __syncthreads();
-    for (int o = 1; o < SCAN_T; o <<= 1) {
+    for (int o = 1; o < NT; o <<= 1) {
         const uint2 y = t >= o ? lds[t - o] : make_uint2(0u, 0u);
         __syncthreads();
         lds[t] = make_uint2(lds[t].x + y.x, lds[t].y + y.y);
         __syncthreads();
     }
-    total = lds[SCAN_T - 1];
+    total = lds[NT - 1];
     const uint2 incl = lds[t];
     __syncthreads();
     return make_uint2(incl.x - x.x, incl.y - x.y);
 }
 
 // Tile sums (order-free, so the loads run across the workgroup: item k * SCAN_T + t).
-__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint32_t *a, const uint32_t *b, long long n,
-                                                       uint32_t *ta, uint32_t *tb) {
+__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint8_t *cs, const uint8_t *ntri, const uint32_t *b,
+                                                       long long n, uint32_t *ta, uint32_t *tb) {
     __shared__ uint2 lds[SCAN_T];
+    __shared__ uint32_t nidx[256];
+    nidx[threadIdx.x] = 3u * ntri[threadIdx.x];
+    __syncthreads();
     const long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint2 s = make_uint2(0u, 0u);
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++) {
         const long long i = base + (long long)k * SCAN_T;
         if (i < n) {
-            s.x += a[i];
+            s.x += nidx[cs[i]];
             s.y += b[i];
         }
     }
@@ -271,14 +276,16 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint32_t *a, const 
 }
 
 // One workgroup: the tile sums become exclusive tile offsets; tot[0..1] the totals.
-__global__ __launch_bounds__(SCAN_T) void k_scan_top(uint32_t *ta, uint32_t *tb, long long ntiles, uint32_t *tot) {
-    __shared__ uint2 lds[SCAN_T];
+constexpr int SCAN_TOP_T = 1024;
+__global__ __launch_bounds__(SCAN_TOP_T) void k_scan_top(uint32_t *ta, uint32_t *tb, long long ntiles,
+                                                         uint32_t *tot) {
+    __shared__ uint2 lds[SCAN_TOP_T];
     uint2 run = make_uint2(0u, 0u);
-    for (long long base = 0; base < ntiles; base += SCAN_T) {
+    for (long long base = 0; base < ntiles; base += SCAN_TOP_T) {
         const long long i = base + threadIdx.x;
         const uint2 x = i < ntiles ? make_uint2(ta[i], tb[i]) : make_uint2(0u, 0u);
         uint2 total;
-        const uint2 ex = block_exclusive(x, lds, total);
+        const uint2 ex = block_exclusive<SCAN_TOP_T>(x, lds, total);
         if (i < ntiles) {
             ta[i] = run.x + ex.x;
             tb[i] = run.y + ex.y;
@@ -293,15 +300,19 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_top(uint32_t *ta, uint32_t *tb,
 
 // In-place exclusive offsets of one tile: staged through LDS so that global loads and
 // stores run across the workgroup while each thread scans SCAN_PER consecutive items.
-__global__ __launch_bounds__(SCAN_T) void k_scan_apply(uint32_t *a, uint32_t *b, long long n, const uint32_t *ta,
+__global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint8_t *cs, const uint8_t *ntri, uint32_t *a,
+                                                       uint32_t *b, long long n, const uint32_t *ta,
                                                        const uint32_t *tb) {
     __shared__ uint2 lds[SCAN_T];
     __shared__ uint32_t sa[SCAN_TILE], sb[SCAN_TILE];
+    __shared__ uint32_t nidx[256];
+    nidx[threadIdx.x] = 3u * ntri[threadIdx.x];
+    __syncthreads();
     const long long tile = (long long)blockIdx.x * SCAN_TILE;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++) {
         const int j = k * SCAN_T + threadIdx.x;
-        sa[j] = tile + j < n ? a[tile + j] : 0u;
+        sa[j] = tile + j < n ? nidx[cs[tile + j]] : 0u;
         sb[j] = tile + j < n ? b[tile + j] : 0u;
     }
     __syncthreads();
@@ -668,9 +679,11 @@ hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *color
     if ((e = hipMemsetAsync(m.nact, 0, sizeof(uint32_t), st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_mesh_count, dim3(grid_for(m.U / job.F)), dim3(256), 0, st, m);
     hipLaunchKernelGGL(k_mesh_own, dim3((unsigned)std::min<long long>(grid_for(m.U), 2048)), dim3(256), 0, st, m);
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.cnt_i, m.cnt_v, m.U, ta, tb);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, ta, tb, ntiles, tot);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.cnt_i, m.cnt_v, m.U, ta, tb);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.ucase, m.ntri, m.cnt_v, m.U, ta,
+                       tb);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_TOP_T), 0, st, ta, tb, ntiles, tot);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_T), 0, st, m.ucase, m.ntri, m.cnt_i, m.cnt_v,
+                       m.U, ta, tb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     uint32_t htot[3];
     if ((e = hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
