@@ -154,3 +154,21 @@ def test_mesh_of_a_snapshot_while_steps_run():
     with pytest.raises(L.LeniaHipError, match="not in the snapshot"):
         eng.mesh(wid, view=2, snapshot_slot=1)
     eng.close()
+
+
+@pytest.mark.parametrize("view", [1, 3])
+def test_mesh_with_the_reference_table_bitwise(view):
+    """The reference's own TRIANGLE_TABLE (tests/golden/tri_table.json), passed as the
+    reference integration passes it: GPU mesh bit-identical to the checker."""
+    import json
+    import os
+    from oracle import oracle as O
+    ref_tab = np.array(json.load(open(os.path.join(os.path.dirname(__file__), "golden", "tri_table.json")))["rows"],
+                       dtype=np.int32)
+    shape, C = (32, 32, 32), 3
+    eng, wid = stepped(shape, C, seed=56)
+    gpu = eng.mesh(wid, view=view, tri_table=ref_tab, colors=COLORS[:C])
+    ref = O.mesh(fields(eng, wid, shape, C, view), ref_tab, colors=COLORS[:C])
+    assert len(gpu[0]) > 100
+    same(gpu, ref)
+    eng.close()

@@ -186,3 +186,28 @@ def test_channels_share_the_edge_map(table):
     assert len(v) == len(v1)  # channel 1 makes no vertex of its own
     assert (v["color"] == 7).all()
     assert np.array_equal(idx, np.concatenate([i1, i1]))
+
+
+def test_reference_table_is_accepted_and_meshes_the_same_surface(table):
+    """The reference's own TRIANGLE_TABLE (fixture written by make_tri_table.py): the
+    same crossing edges as EDGE_TABLE, at most 5 triangles per case, a closed outward
+    sphere, and the very vertices (positions) the default table gives."""
+    ref_tab = np.array(json.load(open(os.path.join(HERE, "golden", "tri_table.json")))["rows"], dtype=np.int32)
+    edges = json.load(open(os.path.join(HERE, "golden", "edge_table.json")))["values"]
+    for m in range(256):
+        row = [int(e) for e in ref_tab[m] if e >= 0]
+        assert len(row) % 3 == 0 and len(row) <= 15 and (ref_tab[m][len(row):] == -1).all()
+        assert sum(1 << e for e in set(row)) == edges[m], m
+    z, y, x = grid(20)
+    c = np.array([9.3, 10.1, 9.7])
+    r = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2)
+    f = np.clip(1 - r / 12, 0, 1).astype(np.float32)[None]
+    v, idx = O.mesh([f], ref_tab)
+    chi, closed = surface_stats(v, idx)
+    assert closed and chi == 2
+    pos = np.stack([v["x"], v["y"], v["z"]], 1)
+    nrm = np.stack([v["nx"], v["ny"], v["nz"]], 1)
+    assert ((pos / 3.5 - c) * nrm).sum(1).min() > 0
+    v2, _ = O.mesh([f], table)
+    key = lambda a: sorted(map(tuple, np.stack([a["x"], a["y"], a["z"]], 1).view(np.uint32).tolist()))
+    assert key(v) == key(v2)
