@@ -211,3 +211,34 @@ def test_reference_table_is_accepted_and_meshes_the_same_surface(table):
     v2, _ = O.mesh([f], table)
     key = lambda a: sorted(map(tuple, np.stack([a["x"], a["y"], a["z"]], 1).view(np.uint32).tolist()))
     assert key(v) == key(v2)
+
+
+VERTEX_PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "lenia_hip.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lh_vertex), offsetof(lh_vertex, x), offsetof(lh_vertex, y),
+           offsetof(lh_vertex, z), offsetof(lh_vertex, nx), offsetof(lh_vertex, ny), offsetof(lh_vertex, nz),
+           offsetof(lh_vertex, color));
+    return 0;
+}
+"""
+
+
+def test_vertex_record_layout_matches_the_c_header(tmp_path):
+    """lh_vertex (include/lenia_hip.h), the Python record and the checker's lo_vertex
+    share the reference Vertex's 28-byte layout: x y z nx ny nz colour."""
+    import subprocess
+    src = tmp_path / "probe.c"
+    src.write_text(VERTEX_PROBE)
+    exe = tmp_path / "probe"
+    try:
+        subprocess.run(["gcc", "-I", os.path.join(HERE, "..", "include"), str(src), "-o", str(exe)], check=True,
+                       capture_output=True)
+    except (OSError, subprocess.CalledProcessError) as e:
+        pytest.skip(f"no C compiler: {e}")
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [E.VERTEX_DTYPE.itemsize] + [E.VERTEX_DTYPE.fields[n][1] for n in ("x", "y", "z", "nx", "ny", "nz", "color")]
+    assert got == want == [28, 0, 4, 8, 12, 16, 20, 24]
+    assert O.VERTEX_DTYPE == E.VERTEX_DTYPE
