@@ -13,23 +13,48 @@
  * fp32 bits.  Two precisions are built from one source: lo_f32_* (float) and
  * lo_f64_* (double, used as the FP64 oracle for error statements).
  *
- * Parity status per function (pinning checks live in tests/test_oracle_*.py):
+ * Parity status per function (pinning checks live in tests/test_oracle_*.py; each one
+ * recomputes the function from the reference's definition in numpy, float64):
  *   fft1 / rfft3 / irfft3  pinned: numpy.fft at 1e-6 (f32) / 1e-12 (f64), brute-force DFT,
- *                          delta -> ones, Parseval, linearity.
+ *                          delta -> ones, Parseval, linearity (test_oracle_fft.py).
  *   kernel_field/spectrum  pinned: sum = 1, torus-even symmetry, real spectrum, K(0)=1,
  *                          closed-form ring profile at sampled radii.
  *   bell LUT               pinned: closed form exp(-((u-m)/s)^2/2) per entry.
- *   Sobel / central diff   pinned: linear fields (axis permutation, x32 scale), torus wrap.
- *   RT / SRT / ADV         pinned: identity at zero displacement (sigma=0.5), integer
- *                          translations, weight sum sn*(2 sigma)^3, mass conservation in
- *                          physics mode, brute-force box-overlap integral on 4^3.
+ *   phase_growth           pinned (test_oracle_pins.py): U_k against a numpy circular
+ *                          convolution of A_c0 with the closed-form kernel field at three
+ *                          shapes (octant-mirror indexing, c0 routing, /N) and one cell by
+ *                          direct summation; U_SUM per c1; H = sum h (LUT[(int)(255u)] - A
+ *                          if asymptotic); B = clamp(A + dt H), soft clip through the
+ *                          degree-7 Taylor fastexp; the analytic bell in physics mode.
+ *   phase_blur             pinned: central differences of a numpy circular 5^3 Gaussian
+ *                          blur of H and S at zero, torus and wall pads; the non-blur copy's
+ *                          extent U_S[0, 5P/2) <- H over the padded layout.
+ *   Sobel / central diff   pinned: linear fields (axis permutation, x32 scale), torus wrap,
+ *                          numpy Sobel of S and H (face-only and full pads).
+ *   alpha / F              pinned: F = NU (1 - a) - NA a with PREALPHA written at c*P+i and
+ *                          read at c*N+i, checked against the closed form of a~_1, a~_2 at
+ *                          32^3, C = 3, and without the quirk.
+ *   phase_mu (mu, V)       pinned: both MU_VEL orders, damping, the +-ma clamp and the wall
+ *                          clamp against numpy from F and V; mu from a linear S end to end.
+ *   SRT                    pinned: per-cell closed form, reference and physics sn / clip,
+ *                          channels < 3 under the quirk; the clamp equals the box overlap.
+ *   RT torus / non-torus   pinned: brute-force box-overlap integral on the torus and at
+ *                          zero / wall / periodic-image pads without the distance wrap;
+ *                          identity at sigma = 1/2; mass conservation in physics mode; the
+ *                          reference's x1.3 mass factor at sigma = 0.65.
+ *   ADV                    pinned: integer translations, zero velocity.
+ *   oscillators, TRM       pinned: Laplacian / flow / divergence closed forms; S = STA sum A,
+ *                          GA = (1-GTA) GA + GTA A.
  *   MC case index          pinned: brute force + generated edge table vs the reference's
  *                          EDGE_TABLE fixture (tests/golden/edge_table.json).
  *   mesh (lo_mesh)         pinned: sphere / torus Euler characteristic 2 / 0, closed and
  *                          consistently oriented, outward normals, vertices on a linear
  *                          field's iso-plane, single-cell octahedron, per-field and
  *                          channel-shared edge maps (tests/test_mesh_oracle.py).
- *   full step              pinned through the above plus the f32-vs-f64 self-consistency.
+ *   full step              the composition of the pinned phases above, in the reference's
+ *                          phase order (render.cpp:376-592), plus the f32-vs-f64 check.
+ *   Not reproduced: FFTW's own float rounding (FFTW is absent; the FFT is this checker's
+ *   canonical Stockham order, pinned to numpy above).
  */
 #ifndef LENIA_ORACLE_H
 #define LENIA_ORACLE_H
