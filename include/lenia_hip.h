@@ -97,9 +97,15 @@ enum {
     LH_OPT_KEEP_USUM = 1,     /* write U_SUM for mesh views (default 0) */
     LH_OPT_USE_GRAPH = 2,     /* replay steps from a captured hipGraph (default 1) */
     LH_OPT_PROFILE = 3,       /* time every kernel launch with HIP events (default 0) */
-    LH_OPT_AUTOTUNE = 4       /* lh_commit times the candidate schedules of each shape class
+    LH_OPT_AUTOTUNE = 4,      /* lh_commit times the candidate schedules of each shape class
                                  (sweep segment lengths, sweep vs tiled transport) on the
                                  class's worlds and keeps the fastest (default 1) */
+    LH_OPT_SCHEDULE = 5       /* forces one schedule on every shape class instead (default -1:
+                                 none): zseg_local | zseg_transport << 8 | tiled << 16, the
+                                 z-planes per sweep workgroup of the gradient and transport
+                                 sweeps (0: the launchers' default, above the depth: the whole
+                                 depth) and 1 to move the sweep's worlds by the tiled kernel.
+                                 Every schedule gives bit-identical results; for tests. */
 };
 
 /* Growth convolution of a world (lh_set_world_conv).  LH_CONV_FFT is the fp32 FFT

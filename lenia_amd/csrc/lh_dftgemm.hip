@@ -32,6 +32,9 @@
 #include <stdint.h>
 #include <string.h>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "lh_common.hpp"
@@ -466,19 +469,27 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2r_growth(PhaseArgs a, 
 }
 
 // A persistent kernel over `total` tiles: as many workgroups as are resident at once (at
-// most one per tile); blocks per CU and CUs are queried once per kernel.
+// most one per tile).  Blocks per CU and CUs are queried once per (kernel, block size,
+// device) and cached under that key: every k_dft_c2c instantiation has the same function
+// type, so the cache cannot live in a per-type static.
+inline int persistent_slots(const void *fn, int threads) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void *, int, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_tuple(fn, threads, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int nb = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, 0) != hipSuccess || nb < 1) nb = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    return cache[key] = nb * cus;
+}
+
 template <class K>
 void launch_persistent(K *fn, int threads, int total, void *st, const PhaseArgs &a, int arg) {
-    static const int slots = [fn, threads] {
-        int nb = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(fn), threads, 0) !=
-                hipSuccess || nb < 1)
-            nb = 1;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-            cus = 256;
-        return nb * cus;
-    }();
+    const int slots = persistent_slots(reinterpret_cast<const void *>(fn), threads);
     const int g = total < slots ? total : slots;
     if (g > 0) hipLaunchKernelGGL(fn, dim3(g), dim3(threads), 0, (hipStream_t)st, a, arg);
 }

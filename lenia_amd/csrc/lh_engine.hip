@@ -183,6 +183,7 @@ struct Engine {
     int parity = 0;
     bool spectra_dirty = true;
     int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0, opt_autotune = 1;
+    int opt_schedule = -1;  // LH_OPT_SCHEDULE: -1 = autotuner / defaults, else the forced schedule
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     std::vector<std::string> prof_names;
     std::map<std::string, Prof> prof;
@@ -587,6 +588,12 @@ lh_status lh_set_option(void *p, int option, int value) {
         if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
         e->opt_autotune = value != 0;
         return LH_OK;
+    case LH_OPT_SCHEDULE:
+        if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
+        if (value < -1 || (value >> 17) != 0)
+            return fail(LH_EINVAL, "LH_OPT_SCHEDULE: -1, or zseg_local | zseg_transport << 8 | tiled << 16");
+        e->opt_schedule = value;
+        return LH_OK;
     case LH_OPT_USE_GRAPH: e->opt_graph = value != 0; return LH_OK;
     case LH_OPT_PROFILE: e->opt_profile = value != 0; return LH_OK;
     }
@@ -717,7 +724,16 @@ lh_status lh_commit(void *p) {
         }
     }
     HIPCHK(hipMalloc(&e->d_mass, e->worlds.size() * sizeof(double)));
-    if (e->opt_autotune) {
+    if (e->opt_schedule >= 0) {
+        // a forced schedule (any segment length is correct; lengths above the depth are
+        // the whole depth), in place of the autotuner's pick
+        for (ShapeClass &c : e->classes) {
+            const int zl = e->opt_schedule & 0xff, zt = (e->opt_schedule >> 8) & 0xff;
+            c.zseg_local = std::min(zl, c.D);
+            c.tiled_transport = c.any_sweep && ((e->opt_schedule >> 16) & 1);
+            c.zseg_transport = c.tiled_transport ? 0 : std::min(zt, c.D);
+        }
+    } else if (e->opt_autotune) {
         lh_status t = autotune(*e);
         if (t != LH_OK) return t;
         HIPCHK(hipMemsetAsync(e->pool, 0, e->pool_floats * sizeof(float), e->stream));

@@ -648,12 +648,12 @@ hipError_t lh_mesh_run(const MeshJob &job, const int *tri, const uint32_t *color
     char *w = (char *)b.work;
     int8_t htri[256 * 16];
     uint8_t hn[256];
+    // A row ends at its first negative entry (lh_mc_invalid checks exactly that prefix);
+    // whatever a caller's row holds after it is replaced by -1 and never read.
     for (int cs = 0; cs < 256; cs++) {
         int n = 0;
-        for (int i = 0; i < 16; i++) {
-            htri[16 * cs + i] = (int8_t)tri[16 * cs + i];
-            n += tri[16 * cs + i] >= 0;
-        }
+        while (n < 16 && tri[16 * cs + n] >= 0) n++;
+        for (int i = 0; i < 16; i++) htri[16 * cs + i] = (int8_t)(i < n ? tri[16 * cs + i] : -1);
         hn[cs] = (uint8_t)(n / 3);
     }
     uint32_t hcol[LHD_MAX_C];

@@ -19,7 +19,7 @@ class Engine:
               "MU": L.F_MU, "USUM": L.F_USUM}
 
     def __init__(self, device: int = 0, keep_velocity=False, keep_usum=False, graph=True,
-                 profile=False, autotune=True):
+                 profile=False, autotune=True, schedule=None):
         self._L = L.load()
         h = C.c_void_p()
         L.check(self._L.lh_create(device, C.byref(h)), "lh_create")
@@ -30,6 +30,9 @@ class Engine:
         self.set_option(L.OPT_USE_GRAPH, graph)
         self.set_option(L.OPT_PROFILE, profile)
         self.set_option(L.OPT_AUTOTUNE, autotune)
+        if schedule is not None:
+            zl, zt, tiled = schedule  # forced sweep segments (local, transport), tiled transport
+            self.set_option(L.OPT_SCHEDULE, int(zl) | int(zt) << 8 | int(bool(tiled)) << 16)
 
     def set_option(self, opt, value):
         L.check(self._L.lh_set_option(self._h, opt, int(value)), "lh_set_option")

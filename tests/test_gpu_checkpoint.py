@@ -1,8 +1,10 @@
-"""Checkpoints on the GPU: a world saved after some steps and restored into a fresh
-engine (built from the image alone) continues bit for bit as the original, for every
-state field; damaged or mismatched images are refused.
+"""Checkpoints on the GPU: a world's image holds the CPU checker's state (A, GA, S, V)
+at the step it was saved; restored into a fresh engine (built from the image alone) it
+continues bit for bit as the checker and as the uninterrupted GPU run, for every state
+field; damaged or mismatched images are refused.  Layout intent: save_config /
+load_config (config.cpp:75-261).
 
-Bar: bit-identical A, GA, S, V after the continued steps; step counters carried over.
+Bar: bit-identical A, GA, S, V; step counters carried over.
 """
 import numpy as np
 import pytest
@@ -43,8 +45,30 @@ def _engine(ws):
     return eng
 
 
+def _checker(ws, marks):
+    """The CPU checker's state fields of every world after each number of steps in marks."""
+    from oracle import oracle as O
+    fmap = {"A": O.F_A, "GA": O.F_GA, "S": O.F_S, "V": O.F_V}
+    out = {k: {} for k in marks}
+    for i, w in enumerate(ws):
+        ow = O.World(w["shape"], w["C"], w["kernels"], w["cfg"], 32)
+        ow.set(O.F_A, w["A"])
+        done = 0
+        for k in marks:
+            ow.step(k - done)
+            done = k
+            for f, of in fmap.items():
+                out[k][(i, f)] = ow.get(of)
+    return out
+
+
+def _eq(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
 def test_restored_worlds_continue_bitwise():
     ws = _worlds()
+    want = _checker(ws, (7, 12))
     eng = _engine(ws)
     eng.step(7)
     images = [eng.checkpoint(i) for i in range(len(ws))]
@@ -62,16 +86,21 @@ def test_restored_worlds_continue_bitwise():
     for i, img in enumerate(images):
         fresh.restore(i, img)
         assert fresh.steps(i) == 7
-    # the image's fields are what a direct read-back gives
-    ck = read_checkpoint(images[1])
-    assert ck["steps"] == 7 and ck["fields"]["V"].any()
+    # every image holds the checker's state at the save step
+    for i, img in enumerate(images):
+        ck = read_checkpoint(img)
+        assert ck["steps"] == 7
+        for f in STATE:
+            assert _eq(ck["fields"][f].reshape(-1), want[7][(i, f)]), ("image", i, f)
+    assert read_checkpoint(images[1])["fields"]["V"].any()
     fresh.step(5)
     fresh.synchronize()
     for i in range(len(ws)):
         assert fresh.steps(i) == 12
         for f in STATE:
             got = fresh.get(i, f)
-            assert np.array_equal(got.view(np.uint32), ref[(i, f)].view(np.uint32)), (i, f)
+            assert _eq(got, want[12][(i, f)]), ("restored vs checker", i, f)
+            assert _eq(got, ref[(i, f)]), ("restored vs uninterrupted", i, f)
     fresh.close()
 
 

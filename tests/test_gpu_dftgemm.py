@@ -121,6 +121,41 @@ def test_rejects_unsupported():
     eng.close()
 
 
+def test_128_one_and_ten_steps_vs_fp64():
+    """Side 128 takes its own matrix-core code path (two 16-row blocks per wave, a
+    persistent grid).  128^3, C=3, K=9, R=16, analytic growth, mass-conserving flow:
+    U_SUM after one step against the FP64 checker: bf16x3 <= 4e-5, fp16x3 within 1.5x the
+    fp32 FFT path's own error (2.2e-5 here: most of it is the kernel field's fp32
+    normalisation, shared by every path); against the fp32 FFT path on the GPU, bf16x3
+    <= 4e-5 and fp16x3 <= 1e-5; A after 1 and 10 steps within 1.5x the fp32 FFT path's
+    own error against FP64 (+1e-6)."""
+    from oracle import oracle as O
+    w = WL.world((128, 128, 128), 3, 9, 16, seed=128, quirks=0, growth="none")
+    ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 64)
+    ow.set(O.F_A, w["A"])
+    ow.step(1)
+    ref = {"USUM": ow.get(O.F_USUM).reshape(3, -1), "A1": ow.get(O.F_A).reshape(3, -1)}
+    ow.step(9)
+    ref["A10"] = ow.get(O.F_A).reshape(3, -1)
+    print("fp64 checker done", flush=True)
+    got = {}
+    for conv in ("fft", "bf16x3", "fp16x3"):
+        one = one_step(w, conv, fields=("USUM", "A"))
+        ten = one_step(w, conv, fields=("A",), steps=10)
+        got[conv] = dict(USUM=one["USUM"], A1=one["A"], A10=ten["A"])
+        e = {k: maxerr(got[conv][k], ref[k]) for k in ("USUM", "A1", "A10")}
+        print(conv, {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    err = {c: {k: maxerr(got[c][k], ref[k]) for k in ("USUM", "A1", "A10")} for c in got}
+    vs_fft = {c: maxerr(got[c]["USUM"], got["fft"]["USUM"]) for c in ("bf16x3", "fp16x3")}
+    print("U_SUM vs the fp32 FFT path", vs_fft, flush=True)
+    assert err["bf16x3"]["USUM"] <= 4e-5, err
+    assert err["fp16x3"]["USUM"] <= 1.5 * err["fft"]["USUM"], err
+    assert vs_fft["bf16x3"] <= 4e-5 and vs_fft["fp16x3"] <= 1e-5, vs_fft
+    for c in ("bf16x3", "fp16x3"):
+        for k in ("A1", "A10"):
+            assert err[c][k] <= 1.5 * err["fft"][k] + 1e-6, (c, k, err)
+
+
 @pytest.mark.skipif(not os.environ.get("LENIA_PRECISION_128"),
                     reason="128^3 x 50-step FP64 comparison (several minutes of CPU); "
                            "set LENIA_PRECISION_128=1")

@@ -172,3 +172,24 @@ def test_mesh_with_the_reference_table_bitwise(view):
     assert len(gpu[0]) > 100
     same(gpu, ref)
     eng.close()
+
+
+def test_entries_after_a_row_terminator_are_ignored():
+    """A caller's table whose rows hold values after their -1 terminator meshes exactly as
+    the same table padded with -1 (rows end at their first negative entry)."""
+    import json
+    import os
+    ref_tab = np.array(json.load(open(os.path.join(os.path.dirname(__file__), "golden", "tri_table.json")))["rows"],
+                       dtype=np.int32)
+    junk = ref_tab.copy()
+    for cs in range(256):
+        n = int(np.argmax(junk[cs] < 0)) if (junk[cs] < 0).any() else 16
+        if n + 1 < 16:
+            junk[cs, n + 1:] = np.arange(16 - n - 1) % 12 + (cs % 3) * 7  # edges, junk and > 11
+    assert not np.array_equal(junk, ref_tab)
+    shape, C = (32, 32, 32), 3
+    eng, wid = stepped(shape, C, seed=57)
+    a = eng.mesh(wid, view=1, tri_table=ref_tab, colors=COLORS[:C])
+    b = eng.mesh(wid, view=1, tri_table=junk, colors=COLORS[:C])
+    same(a, b)
+    eng.close()

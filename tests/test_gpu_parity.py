@@ -305,3 +305,30 @@ def test_autotuned_and_default_schedules_identical():
         eng.close()
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("zs", [256, 16, 8, 4, 2, 1])
+def test_every_schedule_candidate_identical(zs):
+    """Every schedule the autotuner can pick (gradient and transport sweeps at any segment
+    length -- whole depth, halves, quarters, 8, 4, 2, 1 planes -- and the tiled transport)
+    steps a mixed batch bit-identically to the launchers' default schedule."""
+    from lenia_amd.engine import Engine
+    ws = WL.mixed(n_worlds=8, base_seed=510, asymptotic=ASYM)
+
+    def run(schedule):
+        eng = Engine(autotune=False, schedule=schedule)
+        for w in ws:
+            eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        for i, w in enumerate(ws):
+            eng.set(i, "A", w["A"])
+        eng.step(3)
+        out = [eng.get(i, "A") for i in range(len(ws))]
+        eng.close()
+        return out
+
+    ref = run(None)
+    for schedule in ((zs, zs, False), (zs, 0, True)):
+        print(f"schedule {schedule}", flush=True)
+        for a, b in zip(ref, run(schedule)):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), schedule

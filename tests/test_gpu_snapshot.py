@@ -1,6 +1,8 @@
 """Double-buffered read-backs (lh_snapshot*): a snapshot holds exactly the state after
 the steps enqueued before it, however many steps are enqueued after it before it is
-read; two slots alternate.  Reference: the same worlds stepped and read synchronously.
+read; two slots alternate.  Expected values: the CPU checker's A, H and U_SUM after the
+same number of steps (the main thread's GRID -> RBB, H_RB, U_SUM_RB copies,
+main.cpp:225-236), and the same worlds stepped and read synchronously on the GPU.
 
 Bar: bit-identical A, H, U_SUM."""
 import numpy as np
@@ -27,15 +29,35 @@ def _eq(a, b):
     return np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def _checker_states(ws, marks):
+    """The CPU checker's A, H and U_SUM of every world after each number of steps in marks."""
+    from oracle import oracle as O
+    fmap = {"A": O.F_A, "H": O.F_H, "USUM": O.F_USUM}
+    want = {k: {} for k in marks}
+    for i, w in enumerate(ws):
+        ow = O.World(w["shape"], w["C"], w["kernels"], w["cfg"], 32)
+        ow.set(O.F_A, w["A"])
+        done = 0
+        for k in marks:
+            ow.step(k - done)
+            done = k
+            for f, of in fmap.items():
+                want[k][(i, f)] = ow.get(of)
+    return want
+
+
 def test_snapshots_hold_the_state_of_their_step():
     ws = [WL.plain32(seed=61), WL.world((24, 24, 24), 3, 9, 8, seed=62, mode="srt")]
+    marks = (3, 7, 8, 12)
+    want = _checker_states(ws, marks)
     ref = _engine(ws)
-    want = {}
     done = 0
-    for k in (3, 7, 8, 12):  # states after 3, 7, 8 and 12 steps
+    for k in marks:  # the synchronous GPU run agrees with the checker at every mark
         ref.step(k - done)
         done = k
-        want[k] = {(i, f): ref.get(i, f) for i in range(len(ws)) for f in ("A", "H", "USUM")}
+        for i in range(len(ws)):
+            for f in ("A", "H", "USUM"):
+                assert _eq(ref.get(i, f), want[k][(i, f)]), ("synchronous", k, i, f)
     ref.close()
 
     eng = _engine(ws)
