@@ -166,6 +166,16 @@ lh_status lh_world_moments(void *engine, double *out, int n_worlds);
 lh_status lh_mc_case(void *engine, int world, int field, int channel, float iso,
                      int face_only, uint8_t *out);
 
+/* lh_mc_case with no allocation: the same result, staged in a static device scratch array
+ * of 2 MiB per GPU (held until the process ends; worlds of at most 2,097,152 cells, i.e.
+ * up to 128^3; larger ones give LH_EUNSUPPORTED) instead of memory the engine allocates
+ * on first use.  It calls no hipMalloc / hipFree.  Every engine on one GPU shares that
+ * array, so calls that overlap on one GPU (from several host threads) race.  No
+ * counterpart in the reference: for callers that cannot allocate device memory (tight
+ * serving loops, code that must not call hipMalloc). */
+lh_status lh_mc_case_noalloc(void *engine, int world, int field, int channel, float iso,
+                             int face_only, uint8_t *out);
+
 /* ---- meshes ------------------------------------------------------------------------ */
 /* One mesh vertex: the reference's 28-byte Vertex (render.h:443-501) as the mesh
  * threads fill it -- position, unnormalised accumulated normal, packed colour. */

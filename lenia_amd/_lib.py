@@ -26,7 +26,7 @@ Q_ALL = 0x7FF
 # Every symbol include/lenia_hip.h declares.
 EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commit", "lh_set_field",
            "lh_get_field", "lh_field_device_ptr", "lh_state_updated", "lh_step", "lh_synchronize",
-           "lh_stream", "lh_world_mass", "lh_mc_case", "lh_mc_table", "lh_mesh", "lh_mesh_snapshot", "lh_world_count", "lh_profile_count",
+           "lh_stream", "lh_world_mass", "lh_mc_case", "lh_mc_case_noalloc", "lh_mc_table", "lh_mesh", "lh_mesh_snapshot", "lh_world_count", "lh_profile_count",
            "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version",
            "lh_set_world_conv", "lh_world_moments",
            "lh_checkpoint_size", "lh_checkpoint_save", "lh_checkpoint_load", "lh_world_steps",
@@ -119,6 +119,7 @@ def load():
         "lh_snapshot_ready": (i, [vp, i, C.POINTER(i)]),
         "lh_snapshot_read": (i, [vp, i, i, i, C.POINTER(f), st]),
         "lh_mc_case": (i, [vp, i, i, i, f, i, C.POINTER(C.c_uint8)]),
+        "lh_mc_case_noalloc": (i, [vp, i, i, i, f, i, C.POINTER(C.c_uint8)]),
         "lh_mc_table": (i, [C.POINTER(i)]),
         "lh_mesh": (i, [vp, i, i, f, f, i, C.POINTER(i), C.POINTER(C.c_uint32), vp, C.c_int64, C.POINTER(C.c_uint32),
                         C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

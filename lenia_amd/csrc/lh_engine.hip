@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -36,6 +37,36 @@ lh_status fail(lh_status s, const std::string &msg) {
         hipError_t e_ = (x);                                                              \
         if (e_ != hipSuccess) return fail(LH_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// ---- scratch of the allocation-free entry points --------------------------------------
+// lh_mc_case_noalloc stages its case bytes here instead of in memory it allocates: one
+// static device array, which every GPU the library runs on holds until the process ends.
+// In host code `scratch` only names a host-side stand-in; its device address on a GPU
+// comes from hipGetSymbolAddress with that GPU current (scratch_init, called by lh_create)
+// and is kept per device in g_scratch_ptr.  Host code uses only those addresses.
+constexpr size_t LH_SCRATCH_BYTES = (size_t)2 << 20;  // one byte per cell: up to 128^3 cells
+static __device__ __align__(256) uint8_t scratch[LH_SCRATCH_BYTES];
+constexpr int LH_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[LH_SCRATCH_DEVICES] = {};
+
+// Looks up the scratch array's address on `device`, which must be current.
+lh_status scratch_init(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= LH_SCRATCH_DEVICES) return fail(LH_EUNSUPPORTED, "device index too large");
+    if (g_scratch_ptr[device]) return LH_OK;
+    void *ptr = nullptr;
+    hipError_t r = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (r != hipSuccess || !ptr)
+        return fail(LH_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+    g_scratch_ptr[device] = ptr;
+    return LH_OK;
+}
+
+void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < LH_SCRATCH_DEVICES ? g_scratch_ptr[device] : nullptr;
+}
 
 int64_t align64(int64_t v) { return (v + 63) & ~int64_t(63); }
 
@@ -537,6 +568,7 @@ lh_status lh_create(int device, void **out) {
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(LH_EINVAL, "no such device");
     HIPCHK(hipSetDevice(device));
+    if (lh_status st = scratch_init(device)) return st;
     Engine *e = new Engine();
     e->device = device;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1117,6 +1149,30 @@ lh_status lh_mc_case(void *p, int world, int field, int channel, float iso, int 
     lh_launch_mc(f, hw.shape.W, hw.shape.H, hw.shape.D, iso, face_only, e->d_mc, e->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, e->d_mc, N, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LH_OK;
+}
+
+lh_status lh_mc_case_noalloc(void *p, int world, int field, int channel, float iso, int face_only, uint8_t *out) {
+    Engine *e = as_engine(p);
+    if (!e || !out) return fail(LH_EINVAL, "null argument");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (world < 0 || world >= (int)e->worlds.size()) return fail(LH_EINVAL, "bad world id");
+    HostWorld &hw = e->worlds[world];
+    if (field != LH_F_A && field != LH_F_H && field != LH_F_USUM) return fail(LH_EINVAL, "field must be A, H or U_SUM");
+    if (field == LH_F_USUM && !hw.dev.keep_usum) return fail(LH_ESTATE, "U_SUM needs LH_OPT_KEEP_USUM");
+    if (channel < 0 || channel >= hw.shape.C) return fail(LH_EINVAL, "bad channel");
+    const size_t N = (size_t)hw.shape.W * hw.shape.H * hw.shape.D;
+    if (N > LH_SCRATCH_BYTES)
+        return fail(LH_EUNSUPPORTED, "lh_mc_case_noalloc: world has " + std::to_string(N) +
+                                         " cells, above the scratch limit of " + std::to_string(LH_SCRATCH_BYTES));
+    uint8_t *stage = static_cast<uint8_t *>(scratch_ptr(e->device));
+    if (!stage) return fail(LH_ESTATE, "scratch not initialised on this device");
+    HIPCHK(hipSetDevice(e->device));
+    const float *f = e->pool + field_off(*e, hw, field) + (size_t)channel * N;
+    lh_launch_mc(f, hw.shape.W, hw.shape.H, hw.shape.D, iso, face_only, stage, e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, stage, N, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return LH_OK;
 }

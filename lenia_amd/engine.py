@@ -184,11 +184,13 @@ class Engine:
             out[:, a] = np.mod(ang / (2 * np.pi) * ext[a] - 0.5, ext[a])
         return out
 
-    def mc_case(self, wid, field="A", channel=0, iso=0.5, face_only=True) -> np.ndarray:
+    def mc_case(self, wid, field="A", channel=0, iso=0.5, face_only=True, noalloc=False) -> np.ndarray:
+        """Case index bytes [D][H][W]; noalloc=True calls lh_mc_case_noalloc (static scratch)."""
         (W, H, D), _, _ = self.worlds[wid]
         out = np.empty(W * H * D, dtype=np.uint8)
-        L.check(self._L.lh_mc_case(self._h, wid, self.FIELDS[field], channel, float(iso), int(face_only),
-                                   out.ctypes.data_as(C.POINTER(C.c_uint8))), "lh_mc_case")
+        name = "lh_mc_case_noalloc" if noalloc else "lh_mc_case"
+        L.check(getattr(self._L, name)(self._h, wid, self.FIELDS[field], channel, float(iso), int(face_only),
+                                       out.ctypes.data_as(C.POINTER(C.c_uint8))), name)
         return out.reshape(D, H, W)
 
     def mesh(self, wid, view=1, iso=0.5, cell_size=3.5, face_only=True, tri_table=None, colors=None,
