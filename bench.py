@@ -6,18 +6,29 @@ next to the same engine driving a single world.  One JSON line on stdout.
 
   python bench.py [--steps K] [--warmup W] [--worlds B] [--no-cpu] [--no-profile] [--no-single]
                   [--no-mixed] [--no-precision] [--no-proposal] [--no-checkpoint] [--no-sampler]
-                  [--no-mesh]
+                  [--no-mesh] [--no-big] [--no-traffic]
 
-A second leg runs the mixed-shape batch (equal shares of 24^3, 32^3, 36^3, 44x24x44,
-per-world SRT or RT) and reports it under "mixed_batch"; a third steps one 128^3 world
-with the fp32 FFT convolution and with DFT-as-GEMM on the matrix cores
-("precision_128", with the matrix-core roofline of its dominant pass); a fourth times
-the proposal network ("proposal": scoring 10^6 sampled parameter sets in one call,
-training throughput); "checkpoint" times save and restore of world checkpoint images;
-"random_worlds" times the generate_random-range sampler and the Gaussian seeding;
-"snapshot_readback" reads a snapshot back while later steps run; "mesh" times the
-marching-cubes meshes of views 1 and 3 for a 32^3 and a 128^3 world beside the CPU
-restatement of the mesh thread (parity checked bit for bit).
+"roofline": the whole step's compulsory bytes (8 C N + 4 K Noct per world-step) over the
+dominant kernel's average launch time (HIP events on the engine's stream), against the
+8 TB/s HBM peak; "kernel_local" restates it with that kernel's own compulsory bytes.
+"traffic": HBM bytes per launch of that kernel from two rocprofv3 --pmc child runs of
+this script (FETCH_SIZE, WRITE_SIZE), started before this process touches the GPU and
+scaled by what the counters read for a plain copy of known size.
+"cpu_baseline": the fp32 CPU restatement under oracle/ (gcc -O3 -march=native, built on
+this host), one batch world per host core; "cpu_single_world": one world stepped by 2
+threads (the reference's SIM_THREADS) and by all cores in use.  "parity": the 16 checked
+GPU worlds against that restatement, bit for bit; "fp64_error": the same worlds against
+the FP64 restatement after 1 and warmup+steps steps, in reference and physics mode.
+
+Other legs: "mixed_batch" (equal shares of 24^3, 32^3, 36^3, 44x24x44, per-world SRT or
+RT); "big_96" (one 96^3 world, C=4, K=16, R=12, central differences + blur, advection,
+zero border); "precision_128" (one 128^3 world with the fp32 FFT convolution and with
+DFT-as-GEMM on the matrix cores, with the matrix-core roofline of its dominant pass);
+"proposal" (scoring 10^6 sampled parameter sets in one call, training throughput);
+"checkpoint" (save and restore of world images); "random_worlds" (the generate_random
+range sampler and the Gaussian seeding); "snapshot_readback" (a snapshot read back while
+later steps run); "mesh" (marching-cubes meshes of views 1 and 3 for a 32^3 and a 128^3
+world beside the CPU restatement of the mesh thread, parity checked bit for bit).
 
 A step advances every world by one full simulation step (spectral convolution, growth,
 gradients, force, mu, reintegration, end-of-step transform).  The timed region replays
@@ -27,10 +38,17 @@ brackets each launch with HIP events on the engine's stream.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import multiprocessing as mp
 import os
+import re
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -45,28 +63,88 @@ from lenia_amd import roofline as RF  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 FP32_PEAK_TFS = 157.3  # MI355X FP32 vector, published
 BF16_PEAK_TFS = 2500.0  # MI355X dense bf16/fp16 matrix, published
-PROFILE_SUMMARY = os.path.join(REPO, "profiles", "step_kernels_batch4096_32c3.md")
+# rocprofv3 kernel names of each phase the engine's profile reports
+PHASE_KERNELS = {"fwd_xy": r"k_fwd_xy\b", "spec_z": r"k_spec_z(_split)?\b", "inv_growth": r"k_inv_growth\b",
+                 "local": r"k_local_(sweep|tiled)\b", "transport": r"k_transport_(sweep|tiled)\b",
+                 "inv_blur": r"k_inv_blur\b"}
+COPY_MIB = 256  # calibration copy of the traffic probe
 
 
-def profiled_traffic(kernel, worlds):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
-    (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), valid for the default
-    4096-world workload only; None when absent."""
-    if worlds != 4096 or not os.path.exists(PROFILE_SUMMARY):
-        return None
-    col = None
-    for ln in open(PROFILE_SUMMARY):
-        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
-        if col is None:
-            if cols and cols[0] == "kernel":
-                col = next((j for j, c in enumerate(cols) if c.startswith("HBM GB")), None)
-            continue
-        if len(cols) > col and cols[0] == kernel:
-            try:
-                return float(cols[col]) * 1e9
-            except ValueError:
-                return None
-    return None
+def traffic_probe(n_worlds):
+    """Child of measure_traffic, run under rocprofv3 --pmc: the bench workload stepped
+    twice, one plain copy of COPY_MIB MiB as a marker and calibration, two more steps."""
+    worlds = [WL.plain32(seed=100 + i) for i in range(n_worlds)]
+    eng = Engine(graph=False)
+    for w in worlds:
+        eng.add_world(w["shape"], w["C"], w["kernels"], w["cfg"])
+    eng.commit()
+    for i, w in enumerate(worlds):
+        eng.set(i, "A", w["A"])
+    eng.step(2)
+    eng.synchronize()
+    eng.calibration_copy(COPY_MIB << 20)
+    eng.step(2)
+    eng.synchronize()
+    eng.close()
+
+
+def _counter_rows(outdir):
+    """(dispatch id, kernel name, counter, value) from rocprofv3's counter csv."""
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                keys = {k.lower(): k for k in r}
+                did = r.get(keys.get("dispatch_id", ""), r.get(keys.get("correlation_id", ""), "0"))
+                rows.append((int(did), r[keys["kernel_name"]], r[keys["counter_name"]],
+                             float(r[keys["counter_value"]])))
+    return sorted(rows)
+
+
+def measure_traffic(n_worlds, timeout_s=150):
+    """HBM bytes per launch of each step phase, from two rocprofv3 --pmc runs of
+    traffic_probe (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the
+    counter reads for the plain copy of known size.  Run before this process touches
+    the GPU.  Returns (phase -> bytes per launch, notes) or (None, reason)."""
+    rp = shutil.which("rocprofv3")
+    if not rp:
+        return None, "rocprofv3 not found"
+    per = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="lh_traffic_", dir="/tmp")
+        cmd = [rp, "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", "--worlds", str(n_worlds)]
+        env = dict(os.environ, TMPDIR="/tmp")
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                             start_new_session=True)
+        try:
+            _, err = p.communicate(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.communicate()
+            shutil.rmtree(out, ignore_errors=True)
+            return None, f"rocprofv3 --pmc {ctr} exceeded {timeout_s}s"
+        rows = _counter_rows(out)
+        shutil.rmtree(out, ignore_errors=True)
+        if not rows:
+            tail = err.decode(errors="replace").strip().splitlines()[-1:] if err else []
+            return None, f"rocprofv3 --pmc {ctr}: no counter rows (rc {p.returncode}) {' '.join(tail)}"
+        per[ctr] = rows
+    result, notes = {}, {}
+    for ctr, rows in per.items():
+        copy = [r for r in rows if re.search(r"k_copy\b", r[1]) and r[2] == ctr]
+        if not copy:
+            return None, "calibration copy not found in the counter rows"
+        cid = copy[-1][0]
+        true_kib = COPY_MIB * 1024  # the copy reads and writes COPY_MIB MiB
+        scale = true_kib / copy[-1][3]
+        notes[ctr + "_scale"] = scale
+        after = [r for r in rows if r[0] > cid and r[2] == ctr]
+        for phase, pat in PHASE_KERNELS.items():
+            v = sum(r[3] for r in after if re.search(pat, r[1]))
+            if v:
+                result[phase] = result.get(phase, 0.0) + v * scale * 1024 / 2  # two steps
+    return result, notes
 
 
 def build_engine(worlds, profile=False):
@@ -104,12 +182,30 @@ def _cpu_worker(args):
     return seed, snap, n, time.perf_counter() - t0
 
 
-def cpu_baseline(seeds, check_steps, budget_s=12.0):
-    """The fp32 CPU restatement on the host cores: world i of the GPU batch on core i,
-    each stepped for ~budget_s.  Each worker also returns its world's A after the
-    bench's warmup+timed step count, which the GPU result is compared with bitwise."""
+def _fp64_worker(args):
+    """A of one world after 1 and n steps of the FP64 checker."""
+    seed, physics, n = args
     from oracle import oracle as O
-    O.build()
+    w = fp64_world(seed, physics)
+    ow = O.World(w["shape"], w["C"], w["kernels"], w["cfg"], 64)
+    ow.set(O.F_A, w["A"])
+    ow.step(1)
+    a1 = ow.get(O.F_A)
+    ow.step(n - 1)
+    return seed, physics, a1, ow.get(O.F_A)
+
+
+def fp64_world(seed, physics):
+    """A checked world of the batch; physics=True: the same draws with the reference
+    quirks off (analytic growth bell, mass-conserving flow driven by growth)."""
+    return WL.plain32(seed=seed, quirks=0, growth="none") if physics else WL.plain32(seed=seed)
+
+
+def cpu_baseline(seeds, check_steps, budget_s=12.0):
+    """The fp32 CPU restatement (gcc -O3 -march=native, built on this host) on the host
+    cores: world i of the GPU batch on core i, each stepped for ~budget_s.  Each worker
+    also returns its world's A after the bench's warmup+timed step count, which the GPU
+    result is compared with bitwise."""
     cores = max(1, min(16, os.cpu_count() or 1, len(seeds)))
     with mp.get_context("fork").Pool(cores) as pool:
         res = pool.map(_cpu_worker, [(seeds[i], check_steps, budget_s) for i in range(cores)])
@@ -117,8 +213,71 @@ def cpu_baseline(seeds, check_steps, budget_s=12.0):
     wall = max(r[3] for r in res)
     base = dict(value=total_steps / wall, unit="world-steps/s", cores=cores, kind="port",
                 sample=f"{cores} of the batch's 32^3 C=3 K=9 RT worlds, one per host core, "
-                       f"~{budget_s:.0f}s each (fp32 restatement, gcc -O2, single-threaded per world)")
+                       f"~{budget_s:.0f}s each (fp32 restatement, gcc -O3 -march=native, one thread per world)")
     return base, {r[0]: r[1] for r in res}
+
+
+def cpu_single_world(threads_list, budget_s=5.0):
+    """Config 0 on the CPU restatement: one 32^3 C=3 K=9 world stepped by 2 threads (the
+    reference's SIM_THREADS) and by all host cores in use, ~budget_s each."""
+    from oracle import oracle as O
+    out = {}
+    w = WL.plain32(seed=100)
+    for th in threads_list:
+        O.set_threads(th)
+        ow = O.World(w["shape"], w["C"], w["kernels"], w["cfg"], 32)
+        ow.set(O.F_A, w["A"])
+        ow.step(1)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            ow.step(2)
+            n += 2
+        dt = time.perf_counter() - t0
+        out[f"threads_{th}"] = {"value": n / dt, "unit": "world-steps/s", "threads": th,
+                                "ms_per_step": dt * 1e3 / n}
+    O.set_threads(1)
+    return out
+
+
+def fp64_error_leg(seeds, n_steps):
+    """Per-channel max-abs error of A against the FP64 checker after 1 and n_steps steps,
+    for the batch's checked worlds in the reference mode (growth table) and in physics
+    mode (analytic bell, mass-conserving flow), with the physics mode's FP64 mass drift."""
+    gpu = {}
+    for physics in (False, True):
+        ws = [fp64_world(s, physics) for s in seeds]
+        eng = build_engine(ws)
+        m0 = eng.mass()
+        eng.step(1)
+        a1 = [eng.get(i, "A") for i in range(len(ws))]
+        eng.step(n_steps - 1)
+        an = [eng.get(i, "A") for i in range(len(ws))]
+        m1 = eng.mass()
+        eng.close()
+        gpu[physics] = (a1, an, float(np.max(np.abs(m1 - m0) / m0)))
+    cores = max(1, min(16, os.cpu_count() or 1))
+    jobs = [(s, ph, n_steps) for ph in (False, True) for s in seeds]
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_fp64_worker, jobs)
+    out = {}
+    for physics in (False, True):
+        e1 = np.zeros(3)
+        en = np.zeros(3)
+        for k, s in enumerate(seeds):
+            r = next(r for r in res if r[0] == s and r[1] == physics)
+            g1 = gpu[physics][0][k].reshape(3, -1).astype(np.float64)
+            gn = gpu[physics][1][k].reshape(3, -1).astype(np.float64)
+            e1 = np.maximum(e1, np.abs(g1 - r[2].reshape(3, -1)).max(axis=1))
+            en = np.maximum(en, np.abs(gn - r[3].reshape(3, -1)).max(axis=1))
+        key = "physics" if physics else "reference"
+        out[key] = {"worlds": len(seeds), "steps": [1, n_steps],
+                    "max_abs_per_channel": [e1.tolist(), en.tolist()],
+                    "worst_after_1": float(e1.max()), f"worst_after_{n_steps}": float(en.max())}
+        if physics:
+            out[key]["fp64_mass_drift_rel"] = gpu[physics][2]
+    out["note"] = ("reference mode reads the growth from an 8-bit table, so a 1-ulp difference of u at a "
+                   "bin edge moves H by a whole bin; both modes are chaotic, so differences grow with steps")
+    return out
 
 
 def precision_leg(steps, warmup):
@@ -158,6 +317,40 @@ def precision_leg(steps, warmup):
                          "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS, "traffic": None,
                          "flops_per_launch": flops, "avg_launch_us": avg_s * 1e6},
             "conv_flops_per_step": RF.gemm_step_flops(128, 3, 9),
+            "kernels": {k: {"launches": c, "avg_us": t * 1e3 / c} for k, (c, t) in prof.items()}}
+
+
+def big_leg(steps, warmup):
+    """One 96^3 world (C=4, K=16, R=12, central differences with the blur pass, advection
+    with velocity damping, zero border): the FFT, gradient and advection kernels at
+    HBM-resident size, with the whole step's compulsory bytes over its dominant kernel."""
+    w = WL.big96(seed=0)
+    eng = build_engine([w])
+    dt = timed(eng, steps, warmup)
+    mass = float(eng.mass()[0])
+    eng.close()
+    peng = build_engine([w], profile=True)
+    peng.step(warmup)
+    peng.synchronize()
+    peng.profile_reset()
+    peng.step(steps)
+    peng.synchronize()
+    prof = peng.profile()
+    peng.close()
+    dom = max(prof, key=lambda k: prof[k][1])
+    n, ms = prof[dom]
+    avg_s = ms / n * 1e-3
+    step_bytes = RF.step_bytes((96, 96, 96), 4, 16, v_live=True)
+    achieved = step_bytes / avg_s / 1e9
+    return {"workload": "1 world 96^3, C=4, K=16, R=12, central differences + blur, advection (V, damping 0.9), "
+                        "zero border, growth table",
+            "value": steps / dt, "unit": "world-steps/s", "ms_per_step": dt * 1e3 / steps,
+            "mass_finite": bool(np.isfinite(mass)),
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_step": step_bytes, "avg_launch_us": avg_s * 1e6},
+            "step_roofline": {"achieved_GBs": step_bytes / dt * steps / 1e9,
+                              "frac": step_bytes / dt * steps / 1e9 / HBM_PEAK_GBS},
             "kernels": {k: {"launches": c, "avg_us": t * 1e3 / c} for k, (c, t) in prof.items()}}
 
 
@@ -305,6 +498,11 @@ def main():
                     help="skip timing the random-world sampler (10^6 sets) and seeding (every world)")
     ap.add_argument("--no-mesh", action="store_true",
                     help="skip timing the marching-cubes meshes (32^3 and 128^3, views 1 and 3)")
+    ap.add_argument("--no-big", action="store_true",
+                    help="skip the 96^3 leg (C=4, K=16, R=12, central differences + blur, advection, zero border)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per kernel")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -312,6 +510,14 @@ def main():
         print("bench.py measures one GPU: worlds are independent and would be sharded "
               "world_id mod G across GPUs; multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(args.worlds)
+        return
+    # the counter runs are children started before this process initialises the GPU
+    traffic, traffic_notes = (None, "skipped") if args.no_traffic or args.no_profile else measure_traffic(args.worlds)
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.use_native()  # the checker tuned for this host; the same results as the generic build
 
     worlds = [WL.plain32(seed=100 + i) for i in range(args.worlds)]
     seeds = [100 + i for i in range(min(args.worlds, 16))]
@@ -390,27 +596,42 @@ def main():
         dom = max(prof, key=lambda k: prof[k][1])
         n, ms = prof[dom]
         avg_s = ms / n * 1e-3
-        b = RF.kernel_bytes(dom, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds
-        achieved = b / avg_s / 1e9
+        # the whole step's compulsory bytes (8 C N + 4 K Noct per world-step) over the
+        # dominant kernel's launch time
+        step_bytes = RF.step_bytes((32, 32, 32), 3, 9) * args.worlds
+        achieved = step_bytes / avg_s / 1e9
+        kb = RF.kernel_bytes(dom, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds
         line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                            "traffic": profiled_traffic(dom, args.worlds),
-                            "traffic_source": "profiles/step_kernels_batch4096_32c3.md (rocprofv3 --pmc "
-                                              "FETCH_SIZE / WRITE_SIZE, 2F+W after copy calibration)",
-                            "algorithmic_bytes_per_launch": b, "avg_launch_us": avg_s * 1e6}
+                            "traffic": traffic.get(dom) if traffic else None,
+                            "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this bench, "
+                                               "scaled by the counters of a plain copy"
+                                               if traffic else traffic_notes),
+                            "algorithmic_bytes_per_launch": step_bytes, "avg_launch_us": avg_s * 1e6,
+                            "kernel_local": {"bytes_per_launch": kb, "achieved": kb / avg_s / 1e9,
+                                             "frac": kb / avg_s / 1e9 / HBM_PEAK_GBS}}
+        if traffic:
+            line["traffic_per_launch"] = {k: {"bytes": v, "alg_bytes": RF.kernel_bytes(
+                k, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds if k in RF.KNOWN else None}
+                for k, v in traffic.items()}
+            line["traffic_per_launch"]["calibration"] = traffic_notes
         step_flops = RF.step_flops((32, 32, 32), 3, 9) * args.worlds
         line["step_valu"] = {"canonical_flops_per_step": step_flops,
                              "achieved_TFs": step_flops / (ms_per_step * 1e-3) / 1e12,
                              "frac_of_published_fp32": step_flops / (ms_per_step * 1e-3) / 1e12 / FP32_PEAK_TFS}
-        step_bytes = RF.step_bytes((32, 32, 32), 3, 9) * args.worlds
         line["step_roofline"] = {"algorithmic_bytes_per_step": step_bytes,
                                  "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                                  "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
         line["kernels"] = kernels
 
+    if not args.no_big:
+        line["big_96"] = big_leg(args.steps, args.warmup)
+
     if not args.no_cpu:
         base, cpu_final = cpu_baseline(seeds, args.warmup + args.steps)
         line["cpu_baseline"] = base
+        line["cpu_single_world"] = cpu_single_world([2, max(1, min(16, os.cpu_count() or 1))])
+        line["fp64_error"] = fp64_error_leg(seeds, args.warmup + args.steps)
         bad = sum(int(np.count_nonzero(gpu_final[s].view(np.uint32) != a.view(np.uint32)))
                   for s, a in cpu_final.items())
         line["parity"] = {"worlds_compared": len(cpu_final), "steps": args.warmup + args.steps,

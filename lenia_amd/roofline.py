@@ -59,6 +59,9 @@ def kernel_bytes(kernel: str, shape, C, K, mode="rt", v_live=False, growth_rt=Tr
     raise KeyError(kernel)
 
 
+KNOWN = ("fwd_xy", "spec_z", "inv_growth", "local", "transport")
+
+
 def gemm_pass_flops(kernel: str, n: int, split=True) -> float:
     P = 3 if split else 1
     if kernel in ("dft_fx", "dft_ix_growth"):

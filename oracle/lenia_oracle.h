@@ -146,6 +146,8 @@ enum {
 LO_DECLARE(lo_f32_, float)
 LO_DECLARE(lo_f64_, double)
 double lo_canon_exp(double x);
+/* Host threads for the step's parallel loops (default 1); results do not depend on it. */
+int lo_set_threads(int n);
 
 /* Edge mask of a marching-cubes case, generated from the cube's edge list. */
 int lo_edge_mask(int case_index);

@@ -3,6 +3,15 @@
 #define PFX(n) lo_f32_##n
 #include "lenia_oracle_impl.h"
 
+int lo_nthreads = 1;
+
+/* Threads of the checker's parallel loops (1 = serial); results are identical for any
+ * count.  Returns the count in effect. */
+int lo_set_threads(int n) {
+    lo_nthreads = n < 1 ? 1 : n;
+    return lo_nthreads;
+}
+
 /* Edge mask of a marching-cubes case: edge e is cut iff its two corners lie on
  * different sides of the iso level.  Corner pairs follow the cube numbering of
  * marching_cubes.h:17-22 (a 12-entry edge list that must match the reference). */

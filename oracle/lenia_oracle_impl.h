@@ -17,6 +17,17 @@
 typedef REAL real;
 typedef struct { real r, i; } cpx;
 
+/* Host threads of the loops below (lo_set_threads, default 1).  Only loops whose
+ * iterations write disjoint cells run in parallel, each cell's arithmetic in its own
+ * fixed order, so results do not depend on the thread count.  Reductions (mass, the
+ * kernel normaliser) stay sequential. */
+extern int lo_nthreads;
+#ifdef _OPENMP
+#define LO_PAR _Pragma("omp parallel for schedule(static) num_threads(lo_nthreads) if(lo_nthreads > 1)")
+#else
+#define LO_PAR
+#endif
+
 #define PAD 4
 #define NAME(n) PFX(n)
 
@@ -122,9 +133,10 @@ void NAME(fft1)(int n, real *re_im, int inverse) { fft1c(n, (cpx *)re_im, invers
  * x: rows y, y+1 packed as one complex line; y and z: complex lines.  Unnormalised. */
 static void rfft3c(int W, int H, int D, const real *in, cpx *out) {
     int Wh = W / 2 + 1;
-    cpx line[MAXN];
+    LO_PAR
     for (int z = 0; z < D; z++)
         for (int y = 0; y < H; y += 2) {
+            cpx line[MAXN];
             const real *r0 = in + ((size_t)z * H + y) * W, *r1 = r0 + W;
             for (int x = 0; x < W; x++) { line[x].r = r0[x]; line[x].i = r1[x]; }
             fft1c(W, line, 0);
@@ -135,14 +147,18 @@ static void rfft3c(int W, int H, int D, const real *in, cpx *out) {
                 o1[k].r = (Z.i + P.i) * (real)0.5; o1[k].i = (P.r - Z.r) * (real)0.5;
             }
         }
+    LO_PAR
     for (int z = 0; z < D; z++)
         for (int k = 0; k < Wh; k++) {
+            cpx line[MAXN];
             for (int y = 0; y < H; y++) line[y] = out[((size_t)z * H + y) * Wh + k];
             fft1c(H, line, 0);
             for (int y = 0; y < H; y++) out[((size_t)z * H + y) * Wh + k] = line[y];
         }
+    LO_PAR
     for (int y = 0; y < H; y++)
         for (int k = 0; k < Wh; k++) {
+            cpx line[MAXN];
             for (int z = 0; z < D; z++) line[z] = out[((size_t)z * H + y) * Wh + k];
             fft1c(D, line, 0);
             for (int z = 0; z < D; z++) out[((size_t)z * H + y) * Wh + k] = line[z];
@@ -153,21 +169,26 @@ static void rfft3c(int W, int H, int D, const real *in, cpx *out) {
  * Nyquist bins use real parts only.  Unnormalised.  `in` is consumed (overwritten). */
 static void irfft3c(int W, int H, int D, cpx *in, real *out) {
     int Wh = W / 2 + 1;
-    cpx line[MAXN];
+    LO_PAR
     for (int y = 0; y < H; y++)
         for (int k = 0; k < Wh; k++) {
+            cpx line[MAXN];
             for (int z = 0; z < D; z++) line[z] = in[((size_t)z * H + y) * Wh + k];
             fft1c(D, line, 1);
             for (int z = 0; z < D; z++) in[((size_t)z * H + y) * Wh + k] = line[z];
         }
+    LO_PAR
     for (int z = 0; z < D; z++)
         for (int k = 0; k < Wh; k++) {
+            cpx line[MAXN];
             for (int y = 0; y < H; y++) line[y] = in[((size_t)z * H + y) * Wh + k];
             fft1c(H, line, 1);
             for (int y = 0; y < H; y++) in[((size_t)z * H + y) * Wh + k] = line[y];
         }
+    LO_PAR
     for (int z = 0; z < D; z++)
         for (int y = 0; y < H; y += 2) {
+            cpx line[MAXN];
             const cpx *a = in + ((size_t)z * H + y) * Wh, *b = a + Wh;
             line[0].r = a[0].r; line[0].i = b[0].r;
             line[W / 2].r = a[W / 2].r; line[W / 2].i = b[W / 2].r;
@@ -301,6 +322,7 @@ world *NAME(create)(int W, int H, int D, int C, int K, const lo_params *p, const
     w->Nh = (long)w->Wh * H * D; w->Noct = (long)w->Wh * w->Hh * w->Dh;
     w->p = *p;
     for (int i = 0; i < K; i++) w->kern[i] = k[i];
+    (void)get_plan(W); (void)get_plan(H); (void)get_plan(D);  /* the plan cache is filled serially */
     long N = w->N, P = w->P;
     w->GRID = zalloc(4 * P); w->BACK = zalloc(4 * P); w->GH = zalloc(4 * P);
     w->U_SUM = zalloc(4 * P); w->U_S = zalloc(4 * P); w->TT = zalloc(4 * P);
@@ -462,6 +484,7 @@ static real soft_growth(real g, real h, real dt) {
  * edges and corners untouched (the reference fills only the six faces). */
 static void fill_halo(world *w, real *a, int layers, int kind, int faces_only) {
     int W = w->W, H = w->H, D = w->D;
+    LO_PAR
     for (int z = -layers; z < D + layers; z++)
         for (int y = -layers; y < H + layers; y++)
             for (int x = -layers; x < W + layers; x++) {
@@ -533,6 +556,7 @@ static void phase_growth(world *w) {
         const real *T = (const real *)(w->T + (size_t)w->kern[k].c0 * w->Nh);
         const real *kh = w->khat + (size_t)k * w->Noct;
         real *G = (real *)w->tmp;
+        LO_PAR
         for (int z = 0; z < w->D; z++) {
             int zz = z <= w->D / 2 ? z : w->D - z;
             for (int y = 0; y < w->H; y++) {
@@ -550,6 +574,7 @@ static void phase_growth(world *w) {
     for (int k = 0; k < w->K; k++) {
         int c = w->kern[k].c1;
         real h = (real)w->kern[k].h;
+        LO_PAR
         for (int z = 0; z < w->D; z++)
             for (int y = 0; y < w->H; y++)
                 for (int x = 0; x < w->W; x++) {
@@ -563,6 +588,7 @@ static void phase_growth(world *w) {
     if (w->p.growth_type == LO_GROWTH_RT) {
         real dt = (real)w->p.dt;
         for (int c = 0; c < C; c++)
+            LO_PAR
             for (int z = 0; z < w->D; z++)
                 for (int y = 0; y < w->H; y++)
                     for (int x = 0; x < w->W; x++) {
@@ -590,6 +616,7 @@ static void phase_blur(world *w) {
             gather_interior(w, src, f);
             rfft3c(w->W, w->H, w->D, f, w->tmp);
             real *G = (real *)w->tmp;
+            LO_PAR
             for (int z = 0; z < w->D; z++) {
                 int zz = z <= w->D / 2 ? z : w->D - z;
                 for (int y = 0; y < w->H; y++) {
@@ -603,6 +630,7 @@ static void phase_blur(world *w) {
                 }
             }
             irfft3c(w->W, w->H, w->D, w->tmp, o);
+            LO_PAR
             for (int z = 0; z < w->D; z++)
                 for (int y = 0; y < w->H; y++)
                     for (int x = 0; x < w->W; x++) dst[pidx(w, x, y, z)] = o[idx3(w, x, y, z)] / (real)N;
@@ -670,6 +698,7 @@ static void phase_gradients_force(world *w) {
     int C = w->C, sob = w->p.gradient == LO_GRAD_SOBEL;
     const real *S = sob ? w->SUM : w->SUM_S;
     const real *Hf = sob ? w->GH : w->U_S;
+    LO_PAR
     for (int z = 0; z < w->D; z++)
         for (int y = 0; y < w->H; y++)
             for (int x = 0; x < w->W; x++) {
@@ -687,6 +716,7 @@ static void phase_gradients_force(world *w) {
     real ta = (real)w->p.theta_a;
     int qa = (w->p.quirks & LO_Q_PREALPHA_INDEX) != 0;
     for (int c = 0; c < C; c++)
+        LO_PAR
         for (int z = 0; z < w->D; z++)
             for (int y = 0; y < w->H; y++)
                 for (int x = 0; x < w->W; x++) {
@@ -700,6 +730,7 @@ static void phase_gradients_force(world *w) {
                     if (gi < 4 * N) w->PREALPHA[gi] = rclamp(pg, 0, 1);
                 }
     for (int c = 0; c < C; c++)
+        LO_PAR
         for (int z = 0; z < w->D; z++)
             for (int y = 0; y < w->H; y++)
                 for (int x = 0; x < w->W; x++) {
@@ -719,6 +750,7 @@ static void phase_mu(world *w) {
     int wall = w->p.border == LO_BORDER_WALL;
     real lim[3] = {(real)w->W - sig, (real)w->H - sig, (real)w->D - sig};
     for (int c = 0; c < C; c++)
+        LO_PAR
         for (int z = 0; z < w->D; z++)
             for (int y = 0; y < w->H; y++)
                 for (int x = 0; x < w->W; x++) {
@@ -778,6 +810,7 @@ static void phase_reintegration(world *w) {
     int mode = w->p.mode;
     if (mode == LO_MODE_SRT) {
         for (int c = 0; c < nrt; c++)
+            LO_PAR
             for (int z = 0; z < D; z++)
                 for (int y = 0; y < Hh; y++)
                     for (int x = 0; x < W; x++) {
@@ -801,6 +834,7 @@ static void phase_reintegration(world *w) {
         for (int dz = -dd; dz <= dd; dz++)
             for (int dy = -dd; dy <= dd; dy++)
                 for (int dx = -dd; dx <= dd; dx++)
+                    LO_PAR
                     for (int z = 0; z < D; z++)
                         for (int y = 0; y < Hh; y++)
                             for (int x = 0; x < W; x++) {
@@ -830,6 +864,7 @@ static void phase_reintegration(world *w) {
         int wrap = (w->p.quirks & LO_Q_TRANSPORT_TORUS) || w->p.border == LO_BORDER_TORUS;
         real **vel = w->p.adv_vel ? w->V : w->F;
         for (int c = 0; c < C; c++)
+            LO_PAR
             for (int z = 0; z < D; z++)
                 for (int y = 0; y < Hh; y++)
                     for (int x = 0; x < W; x++) {
@@ -866,6 +901,7 @@ static void phase_reintegration(world *w) {
     /* optional oscillators lenia.cpp:803-911 */
     if (w->p.osc_la || w->p.osc_flow || w->p.osc_div_flow) {
         for (int c = 0; c < C; c++)
+            LO_PAR
             for (int z = 0; z < D; z++)
                 for (int y = 0; y < Hh; y++)
                     for (int x = 0; x < W; x++) {
@@ -905,6 +941,7 @@ static void phase_trm(world *w) {
     real dt = (real)w->p.dt;
     int clamp01 = (w->p.quirks & LO_Q_CLAMP01) != 0;
     for (int c = 0; c < C; c++)
+        LO_PAR
         for (int z = 0; z < w->D; z++)
             for (int y = 0; y < w->H; y++)
                 for (int x = 0; x < w->W; x++) {
@@ -919,6 +956,7 @@ static void phase_trm(world *w) {
                 }
     real gtam = (real)1.0f - (real)w->p.gta, gtas = (real)w->p.gta;
     real stam = (real)1.0f - (real)w->p.sta, stas = (real)w->p.sta;
+    LO_PAR
     for (int z = 0; z < w->D; z++)
         for (int y = 0; y < w->H; y++)
             for (int x = 0; x < w->W; x++) {

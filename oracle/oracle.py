@@ -59,6 +59,22 @@ def build() -> str:
     return _LIB_PATH
 
 
+def use_native() -> str:
+    """Builds the host-tuned copy (-O3 -march=native, same results) on this machine and
+    makes lib() load it; call before the first lib() of the process."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+    _LIB_PATH = os.path.join(_HERE, "build", "native", "liblenia_oracle.so")
+    return _LIB_PATH
+
+
+def set_threads(n: int) -> int:
+    """Host threads of the checker's step loops (results do not depend on it)."""
+    return lib().lo_set_threads(int(n))
+
+
 _lib = None
 
 
@@ -91,6 +107,8 @@ def lib():
             getattr(L, pfx + "blur_field").argtypes = [C.c_int] * 3 + [P]
             getattr(L, pfx + "bell_table").argtypes = [C.c_float, C.c_float, P]
             getattr(L, pfx + "mc_case").argtypes = [C.c_int] * 3 + [P, T, C.c_int, C.POINTER(C.c_uint8)]
+        L.lo_set_threads.argtypes = [C.c_int]
+        L.lo_set_threads.restype = C.c_int
         L.lo_edge_mask.argtypes = [C.c_int]
         L.lo_edge_mask.restype = C.c_int
         L.lo_canon_exp.restype = C.c_double

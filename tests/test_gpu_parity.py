@@ -86,15 +86,23 @@ def test_shapes_bitwise(shape, C, K, R):
     ((64, 64, 64), 2, 4, 6, 2, dict(mode="adv", gradient="central", blur=True, sobel_border="none",
                                     border="none", asymptotic=ASYM)),
     # the 96^3 workload: C=4, K=16, R=12, central differences + blur, advection, zero border
-    ((96, 96, 96), 4, 16, 12, 2, dict(mode="adv", gradient="central", blur=True, sobel_border="none",
-                                      border="none", adv_vel=True, asymptotic=ASYM)),
+    ((96, 96, 96), 4, 16, 12, 10, dict(mode="adv", gradient="central", blur=True, sobel_border="none",
+                                       border="none", adv_vel=True, asymptotic=ASYM)),
     ((128, 128, 128), 3, 9, 16, 1, dict(asymptotic=ASYM)),
 ])
 def test_large_worlds_bitwise(shape, C, K, R, steps, over):
-    """Large worlds run the LDS-cooperative FFT kernels and 32-wide stencil tiles."""
+    """Large worlds run the LDS-cooperative FFT kernels and 32-wide stencil tiles; the
+    96^3 workload (central differences + blur, advection, zero border) for 10 steps."""
+    import os
+    from oracle import oracle as O
     w = WL.world(shape, C, K, R, seed=sum(shape) + steps, **over)
-    fields = ["A", "H", "S"] + (["MU"] if w["cfg"]["mode"] != "adv" else [])
-    r = run_pair(w, steps, fields=fields, keep_velocity=True)
+    fields = ["A", "H", "S"] + (["MU"] if w["cfg"]["mode"] != "adv" else []) + \
+        (["V"] if w["cfg"]["mode"] == "adv" else [])
+    O.set_threads(min(8, os.cpu_count() or 1))  # the checker's result does not depend on it
+    try:
+        r = run_pair(w, steps, fields=fields, keep_velocity=True)
+    finally:
+        O.set_threads(1)
     for f in fields:
         bad, err = mismatch(*r[f])
         assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
