@@ -65,6 +65,7 @@ FP32_PEAK_TFS = 157.3  # MI355X FP32 vector, published
 BF16_PEAK_TFS = 2500.0  # MI355X dense bf16/fp16 matrix, published
 # rocprofv3 kernel names of each phase the engine's profile reports
 PHASE_KERNELS = {"fwd_xy": r"k_fwd_xy\b", "spec_z": r"k_spec_z(_split)?\b", "inv_growth": r"k_inv_growth\b",
+                 "growth_fused": r"k_growth_fused\b",
                  "local": r"k_local_(sweep|tiled)\b", "transport": r"k_transport_(sweep|tiled)\b",
                  "inv_blur": r"k_inv_blur\b"}
 COPY_MIB = 256  # calibration copy of the traffic probe

@@ -101,11 +101,14 @@ enum {
                                  (sweep segment lengths, sweep vs tiled transport) on the
                                  class's worlds and keeps the fastest (default 1) */
     LH_OPT_SCHEDULE = 5       /* forces one schedule on every shape class instead (default -1:
-                                 none): zseg_local | zseg_transport << 8 | tiled << 16, the
-                                 z-planes per sweep workgroup of the gradient and transport
-                                 sweeps (0: the launchers' default, above the depth: the whole
-                                 depth) and 1 to move the sweep's worlds by the tiled kernel.
-                                 Every schedule gives bit-identical results; for tests. */
+                                 none): zseg_local | zseg_transport << 8 | tiled << 16 |
+                                 growth << 17, the z-planes per sweep workgroup of the gradient
+                                 and transport sweeps (0: the launchers' default, above the
+                                 depth: the whole depth), 1 to move the sweep's worlds by the
+                                 tiled kernel, and the spectral growth as two launches (1) or
+                                 one fused launch where the shape's half spectrum fits in LDS
+                                 (2; 0: the default choice).  Every schedule gives
+                                 bit-identical results; for tests. */
 };
 
 /* Growth convolution of a world (lh_set_world_conv).  LH_CONV_FFT is the fp32 FFT

@@ -24,6 +24,7 @@ struct DevWorld {
     uint32_t quirks;
     int dd;            // transport halo depth (int)off_table.dd
     float dt, sigma, ma, sn, clip, si, theta_a, gtam, gtas, stam, stas, damp;
+    float inv_theta;   // 1 / theta_a when theta_a is a power of two (x / theta == x * inv_theta exactly), else 0
     float osc_la_amt, osc_flow_amt, osc_div_flow_amt;
     // float offsets into the pool
     int64_t A, A2, GA, S, V, Bk, Hg, USUM, MU, Fv, Fsp, Gsp, KH, LUT, US, SUMS, BH;
@@ -78,6 +79,7 @@ struct ShapeOps {
     launch_fn local;        // gradients, alpha, F, mu, V
     launch_fn transport;    // SRT / RT / ADV, oscillators, TRM
     launch_fn kspec_z;      // forward z on kernel fields -> real octant
+    launch_fn growth_fused; // spec_z + inv_growth in one launch, or nullptr (spectrum too big for LDS)
 };
 
 const ShapeOps *lh_shape_ops(int W, int H, int D);

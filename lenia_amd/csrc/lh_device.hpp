@@ -187,7 +187,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
         asm volatile("" ::: "memory");
         float nu[3];
         grad(c + 1, nu);
-        float pg = in.aal[c] / w.theta_a;
+        float pg = w.inv_theta != 0.0f ? in.aal[c] * w.inv_theta : in.aal[c] / w.theta_a;
         pg = pg * pg;
         const float al = clamp_ref(pg, 0.0f, 1.0f);
         if (w.need_alpha) {
@@ -529,6 +529,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     const float *srcH = a.pool + (blur_src ? w.US : w.Hg);
     const int nitems = (C + 1) * T::SP;
     int fld[LI], xy[LI], pqxy[LI];
+    const float *fsrc[LI];  // the staged field each item reads (S or H_c)
     bool ok_in[LI], ok_out[LI];
 #pragma unroll
     for (int k = 0; k < LI; k++) {
@@ -543,19 +544,28 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
         fld[k] = f;
         xy[k] = xr + W * yr;
         pqxy[k] = (xr + PADR) + S::PW * (yr + PADR);
+        fsrc[k] = f == 0 ? srcS : srcH + (long long)(f > 0 ? f - 1 : 0) * S::N;
     }
     float pf[LI];
     auto fetch = [&](int zp) {
         int oz;
         const int zr = halo_axis(zp, D, kind, oz);
         const int zoff = W * H * zr, pqz = S::PW * S::PH * (zr + PADR);
+        if (cdcopy) {
 #pragma unroll
-        for (int k = 0; k < LI; k++) {
-            const int f = fld[k];
-            const bool cut = cdcopy && f > 0 && (long long)(pqxy[k] + pqz) + (long long)(f - 1) * S::P >= 5 * S::P / 2;
-            const bool ok = (oz ? ok_out[k] : ok_in[k]) && !cut;
-            const float *src = f == 0 ? srcS : srcH + (long long)(f - 1) * S::N;
-            pf[k] = ok ? src[xy[k] + zoff] : 0.0f;
+            for (int k = 0; k < LI; k++) {
+                const int f = fld[k];
+                const bool cut = f > 0 && (long long)(pqxy[k] + pqz) + (long long)(f - 1) * S::P >= 5 * S::P / 2;
+                const bool ok = (oz ? ok_out[k] : ok_in[k]) && !cut;
+                const float *src = f == 0 ? srcS : srcH + (long long)(f - 1) * S::N;
+                pf[k] = ok ? src[xy[k] + zoff] : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < LI; k++) {
+                const bool ok = oz ? ok_out[k] : ok_in[k];
+                pf[k] = ok ? el(fsrc[k], (unsigned)(xy[k] + zoff)) : 0.0f;
+            }
         }
     };
     auto store = [&](int zp) {

@@ -171,6 +171,7 @@ struct ShapeClass {
     // schedule (lh_commit's autotuner; 0 / false = the launchers' defaults)
     int zseg_local = 0, zseg_transport = 0;
     bool tiled_transport = false;  // the tiled kernel also moves the sweep's worlds
+    bool fused_growth = false;     // growth_fused instead of spec_z + inv_growth
     int conv = LH_CONV_FFT;          // growth convolution of every world in the class
     const GemmOps *gops = nullptr;   // matrix-core passes when conv != LH_CONV_FFT
     unsigned short *d_dft = nullptr; // their DFT matrices
@@ -294,8 +295,12 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "transport", c.ops->transport, a, transport_arg(c));
             continue;
         }
-        run(e, "spec_z", c.ops->spec_z, a, 0);
-        run(e, "inv_growth", c.ops->inv_growth, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8));
+        if (c.fused_growth) {
+            run(e, "growth_fused", c.ops->growth_fused, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8));
+        } else {
+            run(e, "spec_z", c.ops->spec_z, a, 0);
+            run(e, "inv_growth", c.ops->inv_growth, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8));
+        }
         if (c.any_blur) {
             run(e, "fwd_xy_blur", c.ops->fwd_xy, a, SRC_HBLUR | ((c.maxC + 1) << 8));
             run(e, "spec_z_blur", c.ops->spec_z, a, 1);
@@ -373,6 +378,15 @@ DevWorld make_dev(const HostWorld &hw, const Engine &e) {
     d.clip = two_s < 1.0f ? two_s : 1.0f;
     d.si = 0.5f + sigma;
     d.theta_a = p.theta_a;
+    {
+        // dividing by a power of two and multiplying by its exact reciprocal round the
+        // same exact quotient, so the sweep may multiply (one instruction, not ten)
+        int ex = 0;
+        const float inv = 1.0f / p.theta_a;
+        const bool pow2 = std::frexp(std::fabs(p.theta_a), &ex) == 0.5f && std::isfinite(inv) &&
+                          std::frexp(std::fabs(inv), &ex) == 0.5f && (double)inv * (double)p.theta_a == 1.0;
+        d.inv_theta = pow2 ? inv : 0.0f;
+    }
     d.gtam = 1.0f - p.gta; d.gtas = p.gta;
     d.stam = 1.0f - p.sta; d.stas = p.sta;
     d.damp = p.v_damp;
@@ -451,6 +465,13 @@ lh_status autotune(Engine &e) {
             if (t < best) { best = t; pick_local = zs; }
         }
         c.zseg_local = pick_local;
+        if (c.ops->growth_fused && !c.gops) {
+            // the spectral growth as spec_z + inv_growth, or as one fused launch
+            const int garg = (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8);
+            const float split = time_phase(e, c.ops->spec_z, a, 0, reps) + time_phase(e, c.ops->inv_growth, a, garg, reps);
+            const float fused = time_phase(e, c.ops->growth_fused, a, garg, reps);
+            c.fused_growth = fused < split;
+        }
         if (c.any_sweep) {
             best = 1e30f;
             int pick = 0;
@@ -622,8 +643,8 @@ lh_status lh_set_option(void *p, int option, int value) {
         return LH_OK;
     case LH_OPT_SCHEDULE:
         if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
-        if (value < -1 || (value >> 17) != 0)
-            return fail(LH_EINVAL, "LH_OPT_SCHEDULE: -1, or zseg_local | zseg_transport << 8 | tiled << 16");
+        if (value < -1 || (value >> 19) != 0 || ((value >> 17) & 3) == 3)
+            return fail(LH_EINVAL, "LH_OPT_SCHEDULE: -1, or zseg_local | zseg_transport << 8 | tiled << 16 | growth << 17");
         e->opt_schedule = value;
         return LH_OK;
     case LH_OPT_USE_GRAPH: e->opt_graph = value != 0; return LH_OK;
@@ -756,6 +777,8 @@ lh_status lh_commit(void *p) {
         }
     }
     HIPCHK(hipMalloc(&e->d_mass, e->worlds.size() * sizeof(double)));
+    for (ShapeClass &c : e->classes)  // default: spec_z + inv_growth; the autotuner may pick the fused launch
+        c.fused_growth = false;
     if (e->opt_schedule >= 0) {
         // a forced schedule (any segment length is correct; lengths above the depth are
         // the whole depth), in place of the autotuner's pick
@@ -764,6 +787,8 @@ lh_status lh_commit(void *p) {
             c.zseg_local = std::min(zl, c.D);
             c.tiled_transport = c.any_sweep && ((e->opt_schedule >> 16) & 1);
             c.zseg_transport = c.tiled_transport ? 0 : std::min(zt, c.D);
+            const int g = (e->opt_schedule >> 17) & 3;  // 0 default, 1 split, 2 fused where built
+            if (g) c.fused_growth = c.ops->growth_fused && !c.gops && g == 2;
         }
     } else if (e->opt_autotune) {
         lh_status t = autotune(*e);

@@ -370,6 +370,196 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     }
 }
 
+// ------------------------------------------------------------------------------------
+// growth_fused: spec_z and inv_growth in one kernel, for worlds whose half spectrum fits
+// in LDS.  One workgroup per (world, destination channel c1).  For every kernel k with
+// c1(k) = c1, in kernel order: the z-FFT of its source channel's columns, the product
+// with the kernel octant and the inverse z-FFT fill the whole half spectrum in LDS; the
+// inverse y-FFTs and the c2r x-FFTs follow in LDS; each thread adds the growth of its two
+// rows to accumulators it keeps in registers across kernels.  Every line's arithmetic is
+// that of k_spec_z and k_inv_growth in the same order, and each cell's kernels are summed
+// in kernel order, so H, B and U_SUM are bit-identical to that pair; the per-kernel
+// spectra never reach HBM (2 x 8 bytes per half-spectrum bin and kernel saved).
+// LDS: planes at a pitch of 1 mod 16 float2, so the z-columns' writes, the y-lines and
+// the row pairs (taken with z fastest) all read 16 lanes from 16 distinct bank pairs.
+// ------------------------------------------------------------------------------------
+template <int W, int H, int D>
+struct FusedCfg {
+    static constexpr int Wh = W / 2 + 1;
+    static constexpr int COLS = Wh * H;       // z-columns (kx, ky)
+    static constexpr int YL = Wh * D;         // y-lines (kx, z)
+    static constexpr int RPS = H / 2 * D;     // row pairs (y, y+1) of every plane
+    static constexpr int PL = H * Wh;         // float2 per spectrum plane
+    static constexpr int PP = PL + (17 - PL % 16) % 16;  // plane pitch, = 1 mod 16
+    static constexpr int ZP = W * H + 1;      // float pitch of the staged output planes
+    static constexpr int SPEC_BYTES = D * PP * 8;
+    static constexpr int LUT_BYTES = LHD_MAX_K * 256 * 4;
+    static constexpr bool FITS = SPEC_BYTES + LUT_BYTES <= 160 * 1024 && D * ZP * 4 <= SPEC_BYTES;
+    static constexpr int MAXW = COLS > YL ? (COLS > RPS ? COLS : RPS) : (YL > RPS ? YL : RPS);
+    static constexpr int T0 = (MAXW + 63) / 64 * 64;
+    // at most 512 threads: 8 waves, two per SIMD (a ninth wave would put three on one SIMD
+    // and leave the others waiting at every barrier; measured 1.3x slower at 32^3), with
+    // 256 VGPRs each, room for U_SUM's 2W accumulators; the few columns and y-lines past
+    // 512 take a second pass
+    static constexpr int threads(bool) { return T0 > 512 ? 512 : T0; }
+};
+
+// The analytic growth bell out of line, so a loop calling it stays small enough to unroll.
+__device__ __noinline__ float bell_growth_call(float u, float m, float s) { return bell_growth(u, m, s); }
+
+template <int W, int H, int D, int CM, bool KU>
+__global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_fused(PhaseArgs a, int) {
+    using S = Shape<W, H, D>;
+    using F = FusedCfg<W, H, D>;
+    constexpr int T = F::threads(KU), Wh = F::Wh, PP = F::PP;
+    constexpr int RPT = (F::RPS + T - 1) / T;  // row pairs per thread
+    extern __shared__ __attribute__((aligned(16))) float2 fused_lds[];
+    float2 *spec = fused_lds;
+    float *slut = reinterpret_cast<float *>(fused_lds + D * PP);
+    // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, so the CM workgroups of
+    // one world are placed 8 apart -- on one XCD, sharing its L2 for the source spectra
+    constexpr int NX = 8;
+    const int b = blockIdx.x, r = b % NX, m = (b / NX) % CM, g = b / (NX * CM);
+    const int wi = g * NX + r, c1 = m;
+    if (wi >= a.nworlds) return;
+    // the world's scalars, read once (the kernels' parameters are read per kernel through
+    // the pointer: a copy of the whole descriptor indexed by kernel would live in scratch)
+    const DevWorld *wp = a.worlds + a.ids[wi];
+    const int C = wp->C, K = wp->K, growth_type = wp->growth_type;
+    if (c1 >= C) return;
+    const int tid = threadIdx.x;
+    const bool lut = (wp->quirks & LHQ_GROWTH_LUT) != 0;
+    const bool asym = (wp->asym_mask >> c1) & 1, soft = (wp->soft_mask >> c1) & 1;
+    const float dt = wp->dt;
+    const float *Ac = a.pool + (a.parity ? wp->A2 : wp->A) + (long long)c1 * S::N;
+    const long long kh_off = wp->KH, lut_off = wp->LUT;
+    float *Hg = a.pool + wp->Hg + (long long)c1 * S::N, *B = a.pool + wp->Bk + (long long)c1 * S::N;
+    float *US = a.pool + wp->USUM + (long long)c1 * S::N;
+    const float2 *Fsp = reinterpret_cast<const float2 *>(a.pool + wp->Fsp);
+    if (lut)
+        for (int i = tid; i < K * 256; i += T) slut[i] = a.pool[lut_off + i];
+    float h[RPT][2 * W], us[RPT][KU ? 2 * W : 1];
+#pragma unroll
+    for (int j = 0; j < RPT; j++)
+#pragma unroll
+        for (int x = 0; x < 2 * W; x++) {
+            h[j][x] = 0.0f;
+            if (KU) us[j][x] = 0.0f;
+        }
+    for (int k = 0; k < K; k++) {
+        const DevKernel kp = wp->kern[k];
+        if (kp.c1 != c1) continue;
+        const float *oct = a.pool + kh_off + (long long)k * S::Noct;
+        __syncthreads();  // the previous kernel's spectrum (and the table) are no longer read
+        // z-columns: forward z-FFT of the source channel, x kernel octant, inverse z-FFT
+        for (int col = tid; col < F::COLS; col += T) {
+            const int kx = col % Wh, ky = col / Wh;
+            const int kyo = ky <= H / 2 ? ky : H - ky;
+            const float2 *src = Fsp + (long long)kp.c0 * S::Nh + col;
+            float2 t[D];
+#pragma unroll
+            for (int z = 0; z < D; z++) t[z] = src[(long long)z * F::PL];
+            fft_line<D, false>(t, a.twD);
+#pragma unroll
+            for (int z = 0; z < D; z++) {
+                const int zo = z <= D / 2 ? z : D - z;
+                const float kv = oct[(zo * S::Hh + kyo) * Wh + kx];
+                t[z] = make_float2(t[z].x * kv, t[z].y * kv);
+            }
+            fft_line<D, true>(t, a.twD);
+#pragma unroll
+            for (int z = 0; z < D; z++) spec[z * PP + col] = t[z];
+        }
+        __syncthreads();
+        // y-lines: inverse y-FFT in place
+        for (int ln = tid; ln < F::YL; ln += T) {
+            const int kx = ln % Wh, z = ln / Wh;
+            float2 *p = spec + z * PP + kx;
+            float2 line[H];
+#pragma unroll
+            for (int y = 0; y < H; y++) line[y] = p[y * Wh];
+            fft_line<H, true>(line, a.twH);
+#pragma unroll
+            for (int y = 0; y < H; y++) p[y * Wh] = line[y];
+        }
+        __syncthreads();
+        // row pairs (z fastest): c2r along x, then the growth of the pair's 2W cells,
+        // mt_fft_step lenia.cpp:501-538
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            const int rp = tid + j * T;
+            if (rp >= F::RPS) break;
+            const int z = rp % D, y = 2 * (rp / D);
+            const float2 *ra = spec + z * PP + y * Wh, *rb = ra + Wh;
+            float2 line[W];
+            line[0] = make_float2(ra[0].x, rb[0].x);
+            line[W / 2] = make_float2(ra[W / 2].x, rb[W / 2].x);
+#pragma unroll
+            for (int q = 1; q < W / 2; q++) {
+                const float2 av = ra[q], bv = rb[q];
+                line[q] = make_float2(av.x - bv.y, av.y + bv.x);
+                line[W - q] = make_float2(av.x + bv.y, bv.x - av.y);
+            }
+            fft_line<W, true>(line, a.twW);
+            const int i0 = (z * H + y) * W;
+            // two fully unrolled loops (the accumulators must stay in registers): the
+            // table lookup inline, the analytic bell as a call
+            if (lut) {
+#pragma unroll
+                for (int e = 0; e < 2 * W; e++) {
+                    const int x = e % W, r = e / W;  // cell (x, y + r)
+                    const float u = (r == 0 ? line[x].x : line[x].y) / (float)S::N;
+                    int iu = (int)(u * 255.0f);
+                    iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
+                    const float g = slut[k * 256 + iu];
+                    const float hk = asym ? (g - Ac[i0 + r * W + x]) * kp.h : g * kp.h;
+                    if (KU) us[j][e] += u;
+                    h[j][e] += hk;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 2 * W; e++) {
+                    const int x = e % W, r = e / W;
+                    const float u = (r == 0 ? line[x].x : line[x].y) / (float)S::N;
+                    const float g = bell_growth_call(u, kp.m, kp.s);
+                    const float hk = asym ? (g - Ac[i0 + r * W + x]) * kp.h : g * kp.h;
+                    if (KU) us[j][e] += u;
+                    h[j][e] += hk;
+                }
+            }
+        }
+    }
+    // H, B and U_SUM through LDS (planes at a pitch of W*H + 1 floats) so the HBM writes
+    // are whole rows; the spectrum is no longer needed
+    float *stage = reinterpret_cast<float *>(fused_lds);
+#pragma unroll
+    for (int pass = 0; pass < (KU ? 2 : 1); pass++) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            const int rp = tid + j * T;
+            if (rp >= F::RPS) break;
+            const int z = rp % D, y = 2 * (rp / D);
+#pragma unroll
+            for (int e = 0; e < 2 * W; e++) stage[z * F::ZP + y * W + e] = pass == 0 ? h[j][e] : us[j][KU ? e : 0];
+        }
+        __syncthreads();
+        for (int i = tid; i < S::N; i += T) {
+            const int z = i / S::PL, r = i % S::PL;
+            const float v = stage[z * F::ZP + r];
+            if (pass == 0) {
+                Hg[i] = v;
+                if (growth_type == 1) {
+                    const float av = Ac[i];
+                    B[i] = soft ? soft_growth(av, v, dt) : clamp_ref(av + v * dt, 0.0f, 1.0f);
+                }
+            } else {
+                US[i] = v;
+            }
+        }
+    }
+}
+
 // inv_blur: inverse y-FFT + c2r of the C+1 blurred planes; U_S = g/N, SUM_S = g/N
 // (h_sum_blur reintegration.cpp:156-177).
 template <int W, int H, int D>
@@ -489,6 +679,27 @@ struct Launch {
     static void inv_blur(const PhaseArgs &a, int, void *st) {
         hipLaunchKernelGGL((k_inv_blur<W, H, D>), dim3(a.nworlds * D), dim3(256), 0, (hipStream_t)st, a, 0);
     }
+    // spec_z + inv_growth in one launch (FusedCfg::FITS shapes); arg as inv_growth's
+    template <int CM, bool KU>
+    static void growth_fused_cm(const PhaseArgs &a, void *st) {
+        using F = FusedCfg<W, H, D>;
+        constexpr size_t lds = F::SPEC_BYTES + F::LUT_BYTES;
+        // dynamic LDS above 64 KB must be allowed per kernel, once
+        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(k_growth_fused<W, H, D, CM, KU>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)attr;
+        hipLaunchKernelGGL((k_growth_fused<W, H, D, CM, KU>), dim3((a.nworlds + 7) / 8 * 8 * CM), dim3(F::threads(KU)),
+                           lds, (hipStream_t)st, a, 0);
+    }
+    static void growth_fused(const PhaseArgs &a, int arg, void *st) {
+        if constexpr (FusedCfg<W, H, D>::FITS) {
+            const bool ku = arg & 1, c3 = (arg >> 8) <= 3;
+            if (c3 && !ku) growth_fused_cm<3, false>(a, st);
+            else if (c3) growth_fused_cm<3, true>(a, st);
+            else if (!ku) growth_fused_cm<LHD_MAX_C, false>(a, st);
+            else growth_fused_cm<LHD_MAX_C, true>(a, st);
+        }
+    }
     // arg: maxC << 8 | planes per sweep workgroup << 16 (0: sweep_zseg)
     static void local(const PhaseArgs &a, int arg, void *st) {
         if (((arg >> 8) & 0xff) <= 3) {
@@ -531,7 +742,8 @@ struct Launch {
         }
     }
     static constexpr ShapeOps ops() {
-        return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z};
+        return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z,
+                        FusedCfg<W, H, D>::FITS ? &growth_fused : nullptr};
     }
 };
 

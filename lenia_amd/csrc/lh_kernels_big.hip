@@ -310,7 +310,7 @@ struct LaunchBig {
         }
     }
     static constexpr ShapeOps ops() {
-        return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z};
+        return ShapeOps{W, H, D, &fwd_xy, &spec_z, &inv_growth, &inv_blur, &local, &transport, &kspec_z, nullptr};
     }
 };
 

@@ -31,8 +31,11 @@ class Engine:
         self.set_option(L.OPT_PROFILE, profile)
         self.set_option(L.OPT_AUTOTUNE, autotune)
         if schedule is not None:
-            zl, zt, tiled = schedule  # forced sweep segments (local, transport), tiled transport
-            self.set_option(L.OPT_SCHEDULE, int(zl) | int(zt) << 8 | int(bool(tiled)) << 16)
+            # forced sweep segments (local, transport), tiled transport, growth (0 default,
+            # 1 spec_z + inv_growth, 2 fused)
+            zl, zt, tiled = schedule[:3]
+            growth = schedule[3] if len(schedule) > 3 else 0
+            self.set_option(L.OPT_SCHEDULE, int(zl) | int(zt) << 8 | int(bool(tiled)) << 16 | int(growth) << 17)
 
     def set_option(self, opt, value):
         L.check(self._L.lh_set_option(self._h, opt, int(value)), "lh_set_option")

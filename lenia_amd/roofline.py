@@ -50,6 +50,9 @@ def kernel_bytes(kernel: str, shape, C, K, mode="rt", v_live=False, growth_rt=Tr
         return C * Nh * C64 + K * Noct * F32 + K * Nh * C64
     if kernel == "inv_growth":
         return K * Nh * C64 + K * 256 * F32 + C * N * F32 + C * N * F32 * (2 if growth_rt else 1)
+    if kernel == "growth_fused":
+        # spec_z + inv_growth in one launch: the channels' half spectra in, growth out
+        return C * Nh * C64 + K * Noct * F32 + K * 256 * F32 + C * N * F32 + C * N * F32 * (2 if growth_rt else 1)
     if kernel == "local":
         v = 6 * C * N * F32 if v_live else 0
         return N * F32 + 2 * C * N * F32 + 3 * C * N * F32 + v
@@ -59,7 +62,7 @@ def kernel_bytes(kernel: str, shape, C, K, mode="rt", v_live=False, growth_rt=Tr
     raise KeyError(kernel)
 
 
-KNOWN = ("fwd_xy", "spec_z", "inv_growth", "local", "transport")
+KNOWN = ("fwd_xy", "spec_z", "inv_growth", "growth_fused", "local", "transport")
 
 
 def gemm_pass_flops(kernel: str, n: int, split=True) -> float:

@@ -7,7 +7,7 @@ from lenia_amd import workloads as WL
 
 
 def run_pair(world: dict, steps: int, fields=("A",), keep_velocity=True, keep_usum=False,
-             graph=True, prec=32):
+             graph=True, prec=32, schedule=None):
     from oracle import oracle as O
     from lenia_amd.engine import Engine
 
@@ -15,7 +15,7 @@ def run_pair(world: dict, steps: int, fields=("A",), keep_velocity=True, keep_us
     ow.set(O.F_A, world["A"])
     if "V" in world:
         ow.set(O.F_V, world["V"])
-    eng = Engine(keep_velocity=keep_velocity, keep_usum=keep_usum, graph=graph)
+    eng = Engine(keep_velocity=keep_velocity, keep_usum=keep_usum, graph=graph, schedule=schedule)
     wid = eng.add_world(world["shape"], world["C"], world["kernels"], world["cfg"])
     eng.commit()
     eng.set(wid, "A", world["A"])

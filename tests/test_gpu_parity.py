@@ -68,6 +68,38 @@ def test_ten_steps_bitwise(name):
         assert bad == 0, f"{name}: {f} after 10 steps differs in {bad} values (max abs {err:.3e})"
 
 
+FUSED = (0, 0, False, 2)  # forced schedule: launchers' sweep defaults, spectral growth fused
+
+
+@pytest.mark.parametrize("name", ["rt_sobel_torus", "rt_asymptotic", "soft_clip", "physics_rt_growth",
+                                  "cd_blur_zero", "growth_transposition", "srt"])
+def test_fused_growth_bitwise(name):
+    """The spectral growth as one fused launch (spec_z + inv_growth with the per-kernel
+    spectra kept in LDS): every field bit-identical to the checker after 3 steps, with the
+    growth table and the analytic bell, asymptotic and soft-clip channels, U_SUM kept."""
+    w = WL.plain32(**CASES[name])
+    fields = ["A", "GA", "S", "H", "USUM"] + (["B"] if w["cfg"]["growth"] == "rt" else [])
+    for keep_usum in (False, True):
+        f = fields if keep_usum else [x for x in fields if x != "USUM"]
+        r = run_pair(w, 3, fields=f, keep_usum=keep_usum, schedule=FUSED)
+        for x in f:
+            bad, err = mismatch(*r[x])
+            assert bad == 0, f"{name} keep_usum={keep_usum}: {x} differs in {bad} values (max abs {err:.3e})"
+
+
+@pytest.mark.parametrize("shape,C,K,R", [((8, 8, 8), 3, 4, 2), ((8, 4, 8), 2, 3, 2), ((12, 12, 12), 4, 6, 3),
+                                         ((16, 16, 16), 4, 16, 4), ((16, 8, 16), 3, 5, 3),
+                                         ((24, 24, 24), 3, 9, 6), ((32, 32, 32), 2, 5, 8)])
+def test_fused_growth_shapes_bitwise(shape, C, K, R):
+    """Every shape whose half spectrum fits in LDS, with 2 to 4 channels and up to 16
+    kernels (uneven kernels per destination channel), fused against the checker."""
+    w = WL.world(shape, C, K, R, seed=sum(shape) + 77, asymptotic=[1, 0, 1, 0])
+    r = run_pair(w, 4, fields=["A", "H", "B"], schedule=FUSED)
+    for f in ("A", "H", "B"):
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
+
+
 @pytest.mark.parametrize("shape,C,K,R", [((8, 8, 8), 3, 4, 2), ((8, 4, 8), 2, 3, 2), ((12, 12, 12), 2, 3, 3),
                                          ((16, 16, 16), 4, 6, 4), ((16, 8, 16), 3, 5, 3),
                                          ((24, 24, 24), 3, 9, 6), ((36, 36, 36), 3, 9, 8),
@@ -336,7 +368,7 @@ def test_every_schedule_candidate_identical(zs):
         return out
 
     ref = run(None)
-    for schedule in ((zs, zs, False), (zs, 0, True)):
+    for schedule in ((zs, zs, False, 1), (zs, 0, True, 2)):
         print(f"schedule {schedule}", flush=True)
         for a, b in zip(ref, run(schedule)):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), schedule
