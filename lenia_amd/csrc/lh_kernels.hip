@@ -502,9 +502,11 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
             }
             fft_line<W, true>(line, a.twW);
             const int i0 = (z * H + y) * W;
-            // two fully unrolled loops (the accumulators must stay in registers): the
-            // table lookup inline, the analytic bell as a call
-            if (lut) {
+            // fully unrolled loops without branches inside (the accumulators must stay
+            // in registers, and a branch per cell would serialise the table reads): the
+            // table lookup inline, A of the two rows loaded ahead for an asymptotic
+            // channel, the analytic bell as a call
+            if (lut && !asym) {
 #pragma unroll
                 for (int e = 0; e < 2 * W; e++) {
                     const int x = e % W, r = e / W;  // cell (x, y + r)
@@ -512,7 +514,26 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
                     int iu = (int)(u * 255.0f);
                     iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
                     const float g = slut[k * 256 + iu];
-                    const float hk = asym ? (g - Ac[i0 + r * W + x]) * kp.h : g * kp.h;
+                    const float hk = g * kp.h;
+                    if (KU) us[j][e] += u;
+                    h[j][e] += hk;
+                }
+            } else if (lut) {
+                float av[2 * W];
+                const float4 *a4 = reinterpret_cast<const float4 *>(Ac + i0);
+#pragma unroll
+                for (int q = 0; q < W / 2; q++) {
+                    const float4 v = a4[q];
+                    av[4 * q] = v.x; av[4 * q + 1] = v.y; av[4 * q + 2] = v.z; av[4 * q + 3] = v.w;
+                }
+#pragma unroll
+                for (int e = 0; e < 2 * W; e++) {
+                    const int x = e % W, r = e / W;
+                    const float u = (r == 0 ? line[x].x : line[x].y) / (float)S::N;
+                    int iu = (int)(u * 255.0f);
+                    iu = iu < 0 ? 0 : (iu > 255 ? 255 : iu);
+                    const float g = slut[k * 256 + iu];
+                    const float hk = (g - av[e]) * kp.h;
                     if (KU) us[j][e] += u;
                     h[j][e] += hk;
                 }
