@@ -11,6 +11,10 @@
 __device__ __forceinline__ float fmax_ref(float a, float b) { return a > b ? a : b; }  // _mm_max_ps
 __device__ __forceinline__ float fmin_ref(float a, float b) { return a < b ? a : b; }  // _mm_min_ps
 __device__ __forceinline__ float clamp_ref(float a, float lo, float hi) { return fmin_ref(fmax_ref(a, lo), hi); }
+// clamp_ref in one instruction (v_med3_f32), for lo < hi where it gives the same bits: a
+// NaN gives lo in both; a zero a is returned unchanged by both unless a bound is also a
+// zero of the other sign, so callers use it only where a is never -0 or no bound is 0.
+__device__ __forceinline__ float clamp_med3(float a, float lo, float hi) { return __builtin_amdgcn_fmed3f(a, lo, hi); }
 
 template <int W, int H, int D>
 struct Shape {
@@ -189,7 +193,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
         grad(c + 1, nu);
         float pg = w.inv_theta != 0.0f ? in.aal[c] * w.inv_theta : in.aal[c] / w.theta_a;
         pg = pg * pg;
-        const float al = clamp_ref(pg, 0.0f, 1.0f);
+        const float al = clamp_med3(pg, 0.0f, 1.0f);  // pg is a square: never -0
         if (w.need_alpha) {
             // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
             const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
@@ -218,20 +222,19 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const float src = w.adv_vel ? vn[d] : F[d];
-                el(MU, (unsigned)((3 * c + d) * S::N) + iu) = fmin_ref(fmax_ref(src * dt, nma), ma);
+                el(MU, (unsigned)((3 * c + d) * S::N) + iu) = clamp_med3(src * dt, nma, ma);  // -ma < 0 < ma
             }
         } else {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 float m = (w.mu_vel ? v[d] : F[d]) * dt;
-                m = fmax_ref(m, nma);
-                m = fmin_ref(m, ma);
+                m = clamp_med3(m, nma, ma);  // -ma < 0 < ma
                 m = pos[d] + m;
-                if (wall) { m = fmax_ref(m, sig); m = fmin_ref(m, lim[d]); }
+                if (wall) m = clamp_med3(m, sig, lim[d]);  // 0 < sigma < n - sigma
                 el(MU, (unsigned)((3 * c + d) * S::N) + iu) = m;
             }
         }
-        if (!w.mu_vel) {
+        if (!w.mu_vel && w.v_live) {  // (MU_VEL implies a live V)
 #pragma unroll
             for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
         }
@@ -643,9 +646,10 @@ __device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w,
             const bool soft = (w.soft_mask >> c) & 1;
             if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) el(a.pool + w.Bk, g) = soft_growth(el(Acur, g), el(Hg, g), w.dt);
             else if (soft) t = soft_growth(t, el(Hg, g), w.dt);
-            else t = clamp01 ? clamp_ref(t + el(Hg, g) * w.dt, 0.0f, 1.0f) : t + el(Hg, g) * w.dt;
+            else t = clamp01 ? clamp_med3(t + el(Hg, g) * w.dt, 0.0f, 1.0f) : t + el(Hg, g) * w.dt;
         } else if (clamp01) {
-            t = clamp_ref(t, 0.0f, 1.0f);
+            // TT starts at +0 and only gains products, so it (and TT + dt H) is never -0
+            t = clamp_med3(t, 0.0f, 1.0f);
         }
         ttsum = ttsum + t;
         el(GA, g) = w.gtam * in.ga[c] + t * w.gtas;

@@ -244,6 +244,9 @@ __device__ __forceinline__ void dma_planes(float2 *dst, const float2 *src, long 
     }
 }
 
+// The analytic growth bell out of line, so a loop calling it stays small enough to unroll.
+__device__ __noinline__ float bell_growth_call(float u, float m, float s) { return bell_growth(u, m, s); }
+
 template <int W, int H, int D>
 struct InvCfg {
     static constexpr int PLANE = H * (W / 2 + 1);  // float2 per spectrum plane
@@ -403,9 +406,6 @@ struct FusedCfg {
     // 512 take a second pass
     static constexpr int threads(bool) { return T0 > 512 ? 512 : T0; }
 };
-
-// The analytic growth bell out of line, so a loop calling it stays small enough to unroll.
-__device__ __noinline__ float bell_growth_call(float u, float m, float s) { return bell_growth(u, m, s); }
 
 template <int W, int H, int D, int CM, bool KU>
 __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_fused(PhaseArgs a, int) {
