@@ -109,8 +109,9 @@ __device__ __forceinline__ void fft_line(float2 (&x)[N], const float2 *__restric
 
 // Block-cooperative Stockham over LINES lines of N points held in LDS (line l at
 // a[l * LDA + i]); b is scratch of the same layout.  Every thread of the block calls it;
-// each stage spreads the LINES * N/R butterflies over the block's NT threads and ends
-// with a barrier.  Per butterfly the arithmetic is fft_line's, so a line transformed
+// each stage spreads the LINES * N/R butterflies over the block's NT threads (lines
+// fastest, so with an odd LDA the lanes of a wave read and write distinct LDS banks) and
+// ends with a barrier.  Per butterfly the arithmetic is fft_line's, so a line transformed
 // here matches a register-resident transform bit for bit.  The result is left in a.
 template <int N, bool INV, int LINES, int LDA, int NT, int S, int NS>
 __device__ __forceinline__ void fft_lds_stages(float2 *src, float2 *dst, const float2 *__restrict__ tw) {
@@ -119,7 +120,7 @@ __device__ __forceinline__ void fft_lds_stages(float2 *src, float2 *dst, const f
         constexpr int NR = N / R;
         constexpr int TS = N / (NS * R);
         for (int t = threadIdx.x; t < LINES * NR; t += NT) {
-            const int l = t / NR, j = t % NR;
+            const int l = t % LINES, j = t / LINES;
             const int k = j % NS;
             float2 v[R];
 #pragma unroll
@@ -143,7 +144,7 @@ __device__ __forceinline__ void fft_lds_batch(float2 *a, float2 *b, const float2
     fft_lds_stages<N, INV, LINES, LDA, NT, 0, 1>(a, b, tw);
     if constexpr (fft_nst(N) & 1) {
         for (int t = threadIdx.x; t < LINES * N; t += NT) {
-            const int l = t / N, i = t % N;
+            const int l = t % LINES, i = t / LINES;
             a[l * LDA + i] = b[l * LDA + i];
         }
         __syncthreads();
