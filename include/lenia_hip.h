@@ -149,8 +149,17 @@ lh_status lh_field_device_ptr(void *engine, int world, int field, float **dptr, 
 lh_status lh_state_updated(void *engine);
 
 /* ---- stepping ------------------------------------------------------------------ */
-/* Enqueues nsteps full steps of every world on the engine's stream (asynchronous). */
+/* Enqueues nsteps full steps of every world on the engine's stream (asynchronous).
+ * With graphs on (the default), up to 256 steps of every shape class are one launch of
+ * a captured hipGraph (captured on the first call with that step count and kept; calls
+ * of more steps launch 256-step graphs and one for the remainder).  Replaces the
+ * reference's per-step phase hand-off between its simulation threads
+ * (render.cpp:357-604, main.cpp:173-247). */
 lh_status lh_step(void *engine, int nsteps);
+/* Captures the graphs lh_step(engine, nsteps) will launch (1 <= nsteps <= 256, both A
+ * buffer parities) without running them, so a later lh_step of that count starts at
+ * once.  No reference counterpart (graph capture is set-up work). */
+lh_status lh_prepare_steps(void *engine, int nsteps);
 lh_status lh_synchronize(void *engine);
 /* The engine's hipStream_t. */
 void *lh_stream(void *engine);

@@ -25,7 +25,7 @@ Q_ALL = 0x7FF
 
 # Every symbol include/lenia_hip.h declares.
 EXPORTS = ("lh_create", "lh_destroy", "lh_set_option", "lh_add_world", "lh_commit", "lh_set_field",
-           "lh_get_field", "lh_field_device_ptr", "lh_state_updated", "lh_step", "lh_synchronize",
+           "lh_get_field", "lh_field_device_ptr", "lh_state_updated", "lh_step", "lh_prepare_steps", "lh_synchronize",
            "lh_stream", "lh_world_mass", "lh_mc_case", "lh_mc_case_noalloc", "lh_mc_table", "lh_mesh", "lh_mesh_snapshot", "lh_world_count", "lh_profile_count",
            "lh_profile_entry", "lh_profile_reset", "lh_calibration_copy", "lh_shape_supported", "lh_last_error", "lh_version",
            "lh_set_world_conv", "lh_world_moments",
@@ -105,6 +105,7 @@ def load():
         "lh_field_device_ptr": (i, [vp, i, i, C.POINTER(C.POINTER(f)), C.POINTER(st)]),
         "lh_state_updated": (i, [vp]),
         "lh_step": (i, [vp, i]),
+        "lh_prepare_steps": (i, [vp, i]),
         "lh_synchronize": (i, [vp]),
         "lh_stream": (vp, [vp]),
         "lh_world_mass": (i, [vp, C.POINTER(C.c_double), i]),

@@ -216,7 +216,15 @@ struct Engine {
     bool spectra_dirty = true;
     int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0, opt_autotune = 1;
     int opt_schedule = -1;  // LH_OPT_SCHEDULE: -1 = autotuner / defaults, else the forced schedule
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // captured step sequences, keyed by (starting parity, steps): one graph launch
+    // advances every shape class by that many steps; least recently used evicted
+    struct StepGraph {
+        int parity, steps;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<StepGraph> graphs;
+    uint64_t graph_clock = 0;
     std::vector<std::string> prof_names;
     std::map<std::string, Prof> prof;
 };
@@ -558,21 +566,41 @@ lh_status build_kernel_spectra(Engine &e) {
 }
 
 void destroy_graphs(Engine &e) {
-    for (int i = 0; i < 2; i++)
-        if (e.graph[i]) {
-            (void)hipGraphExecDestroy(e.graph[i]);
-            e.graph[i] = nullptr;
-        }
+    for (Engine::StepGraph &g : e.graphs) (void)hipGraphExecDestroy(g.exec);
+    e.graphs.clear();
 }
 
-lh_status capture(Engine &e, int parity) {
+// Longest step sequence captured as one graph, and how many graphs are kept.
+constexpr int kMaxGraphSteps = 256;
+constexpr size_t kMaxGraphs = 6;
+
+// The graph advancing every world by `steps` steps from A-buffer parity `parity`,
+// captured on first use (the kernels of all shape classes, step after step, on the
+// engine stream).
+lh_status step_graph(Engine &e, int parity, int steps, hipGraphExec_t *out) {
+    for (Engine::StepGraph &g : e.graphs)
+        if (g.parity == parity && g.steps == steps) {
+            g.used = ++e.graph_clock;
+            *out = g.exec;
+            return LH_OK;
+        }
+    if (e.graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(e.graphs.begin(), e.graphs.end(),
+                                    [](const Engine::StepGraph &x, const Engine::StepGraph &y) { return x.used < y.used; });
+        HIPCHK(hipStreamSynchronize(e.stream));  // a launched graph may still be running
+        (void)hipGraphExecDestroy(lru->exec);
+        e.graphs.erase(lru);
+    }
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(e, parity);
-    enqueue_step(e, parity ^ 1);
+    for (int s = 0; s < steps; s++) enqueue_step(e, parity ^ (s & 1));
     HIPCHK(hipStreamEndCapture(e.stream, &g));
-    HIPCHK(hipGraphInstantiate(&e.graph[parity], g, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ie);
+    e.graphs.push_back(Engine::StepGraph{parity, steps, ex, ++e.graph_clock});
+    *out = ex;
     return LH_OK;
 }
 
@@ -868,13 +896,16 @@ lh_status lh_step(void *p, int nsteps) {
     for (HostWorld &w : e->worlds) w.steps += (uint64_t)nsteps;
     int left = nsteps;
     if (e->opt_graph && !e->opt_profile) {
-        while (left >= 2) {
-            if (!e->graph[e->parity]) {
-                lh_status s = capture(*e, e->parity);
-                if (s != LH_OK) return s;
-            }
-            HIPCHK(hipGraphLaunch(e->graph[e->parity], e->stream));
-            left -= 2;
+        // one graph launch per kMaxGraphSteps steps (a call of up to that many steps is
+        // a single launch)
+        while (left > 0) {
+            const int n = left < kMaxGraphSteps ? left : kMaxGraphSteps;
+            hipGraphExec_t ex = nullptr;
+            lh_status s = step_graph(*e, e->parity, n, &ex);
+            if (s != LH_OK) return s;
+            HIPCHK(hipGraphLaunch(ex, e->stream));
+            e->parity ^= n & 1;
+            left -= n;
         }
     }
     for (; left > 0; left--) {
@@ -882,6 +913,21 @@ lh_status lh_step(void *p, int nsteps) {
         e->parity ^= 1;
     }
     HIPCHK(hipGetLastError());
+    return LH_OK;
+}
+
+lh_status lh_prepare_steps(void *p, int nsteps) {
+    Engine *e = as_engine(p);
+    if (!e) return fail(LH_EINVAL, "null engine");
+    if (!e->committed) return fail(LH_ESTATE, "commit first");
+    if (nsteps < 1 || nsteps > kMaxGraphSteps) return fail(LH_EINVAL, "nsteps outside [1, 256]");
+    if (!e->opt_graph || e->opt_profile) return LH_OK;
+    HIPCHK(hipSetDevice(e->device));
+    hipGraphExec_t ex = nullptr;
+    for (int parity = 0; parity < 2; parity++) {
+        lh_status s = step_graph(*e, parity, nsteps, &ex);
+        if (s != LH_OK) return s;
+    }
     return LH_OK;
 }
 

@@ -155,7 +155,12 @@ class Engine:
         L.check(self._L.lh_state_updated(self._h), "lh_state_updated")
 
     def step(self, n=1):
+        """Enqueues n steps (up to 256 steps are one captured-graph launch)."""
         L.check(self._L.lh_step(self._h, int(n)), "lh_step")
+
+    def prepare_steps(self, n):
+        """Captures the graphs step(n) launches, so the first such call starts at once."""
+        L.check(self._L.lh_prepare_steps(self._h, int(n)), "lh_prepare_steps")
 
     def synchronize(self):
         L.check(self._L.lh_synchronize(self._h), "lh_synchronize")

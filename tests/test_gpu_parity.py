@@ -227,6 +227,62 @@ def test_batched_mixed_shapes_match_single_worlds():
         assert bad == 0, (i, w["shape"], w["cfg"]["mode"], bad, err)
 
 
+def test_mixed_batch_4096_hundred_steps_one_launch():
+    """4096 worlds of the four mixed shapes with per-world SRT/RT, advanced 100 steps by
+    one call (one captured-graph launch for all shape classes); worlds of every shape and
+    both modes spot-checked against the CPU checker bit for bit, and a graph of 100
+    steps equals 100 single eager steps on a small mixed batch."""
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    ws = WL.mixed(n_worlds=4096, base_seed=5000, asymptotic=ASYM)
+    eng = Engine()
+    for w in ws:
+        eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    for i, w in enumerate(ws):
+        eng.set(i, "A", w["A"])
+    eng.prepare_steps(100)
+    eng.step(100)
+    eng.synchronize()
+    picks = []
+    for s in range(4):  # the first SRT and the first RT world of each shape
+        for mode in ("srt", "rt"):
+            picks.append(next(i for i in range(s, 4096, 4) if ws[i]["cfg"]["mode"] == mode))
+    picks.append(4095)
+    got = {i: eng.get(i, "A") for i in picks}
+    eng.close()
+    O.set_threads(8)
+    try:
+        for i in picks:
+            w = ws[i]
+            ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 32)
+            ow.set(O.F_A, w["A"])
+            ow.step(100)
+            bad, err = mismatch(got[i], ow.get(O.F_A))
+            assert bad == 0, (i, w["shape"], w["cfg"]["mode"], bad, err)
+    finally:
+        O.set_threads(1)
+    small = WL.mixed(n_worlds=8, base_seed=5100, asymptotic=ASYM)
+    outs = []
+    for graph in (True, False):
+        eng = Engine(graph=graph)
+        for w in small:
+            eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        for i, w in enumerate(small):
+            eng.set(i, "A", w["A"])
+        if graph:
+            eng.step(100)
+        else:
+            for _ in range(100):
+                eng.step(1)
+        eng.synchronize()
+        outs.append([eng.get(i, "A") for i in range(len(small))])
+        eng.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_large_batch_one_thread_per_column_spectral_path():
     """128 worlds of 32^3 (69,632 spectral columns) take spec_z's one-thread-per-column
     schedule; smaller batches take the per-(column, kernel) one.  Spot-checked worlds of
