@@ -87,7 +87,8 @@ enum {
     LH_F_V = 3,    /* V_x,V_y,V_z            [C][3][N] (state) */
     LH_F_H = 4,    /* growth H of last step  [C][N] */
     LH_F_B = 5,    /* BACK of last step      [C][N] */
-    LH_F_MU = 6,   /* MU (RT/SRT) or clamped displacement (ADV) [C][3][N] */
+    LH_F_MU = 6,   /* MU (RT/SRT) or clamped displacement (ADV) [C][3][N]
+                      (for SRT worlds without oscillators needs LH_OPT_KEEP_MU) */
     LH_F_USUM = 7  /* U_SUM of last step     [C][N] (needs LH_OPT_KEEP_USUM) */
 };
 
@@ -100,7 +101,7 @@ enum {
     LH_OPT_AUTOTUNE = 4,      /* lh_commit times the candidate schedules of each shape class
                                  (sweep segment lengths, sweep vs tiled transport) on the
                                  class's worlds and keeps the fastest (default 1) */
-    LH_OPT_SCHEDULE = 5       /* forces one schedule on every shape class instead (default -1:
+    LH_OPT_SCHEDULE = 5,      /* forces one schedule on every shape class instead (default -1:
                                  none): zseg_local | zseg_transport << 8 | tiled << 16 |
                                  growth << 17, the z-planes per sweep workgroup of the gradient
                                  and transport sweeps (0: the launchers' default, above the
@@ -109,6 +110,9 @@ enum {
                                  one fused launch where the shape's half spectrum fits in LDS
                                  (2; 0: the default choice).  Every schedule gives
                                  bit-identical results; for tests. */
+    LH_OPT_KEEP_MU = 6        /* write MU for every world (default 0: SRT worlds without
+                                 oscillators are moved inside the gradient sweep from each
+                                 cell's own mu, which then never reaches HBM) */
 };
 
 /* Growth convolution of a world (lh_set_world_conv).  LH_CONV_FFT is the fp32 FFT

@@ -15,7 +15,7 @@ MAX_K, MAX_C, MAX_BETA = 16, 4, 12
 
 OK, EINVAL, EUNSUPPORTED, EHIP, ESTATE, ENOMEM = 0, -1, -2, -3, -4, -5
 F_A, F_GA, F_S, F_V, F_H, F_B, F_MU, F_USUM = range(8)
-OPT_KEEP_VELOCITY, OPT_KEEP_USUM, OPT_USE_GRAPH, OPT_PROFILE, OPT_AUTOTUNE, OPT_SCHEDULE = range(6)
+OPT_KEEP_VELOCITY, OPT_KEEP_USUM, OPT_USE_GRAPH, OPT_PROFILE, OPT_AUTOTUNE, OPT_SCHEDULE, OPT_KEEP_MU = range(7)
 
 MODE = {"srt": 0, "rt": 1, "adv": 2}
 BORDER = {"torus": 0, "wall": 1, "none": 2}

@@ -18,7 +18,7 @@ struct DevWorld {
     int W, H, D, C, K;
     int mode, border, sobel_border, gradient, growth_type;
     int mu_vel, adv_vel, blur;
-    int keep_v, keep_usum, v_live, need_f, need_alpha;
+    int keep_v, keep_usum, keep_mu, v_live, need_f, need_alpha;
     int asym_mask, soft_mask;
     int osc_la, osc_flow, osc_div_flow, osc_vel;
     uint32_t quirks;
@@ -27,7 +27,8 @@ struct DevWorld {
     float inv_theta;   // 1 / theta_a when theta_a is a power of two (x / theta == x * inv_theta exactly), else 0
     float osc_la_amt, osc_flow_amt, osc_div_flow_amt;
     // float offsets into the pool
-    int64_t A, A2, GA, S, V, Bk, Hg, USUM, MU, Fv, Fsp, Gsp, KH, LUT, US, SUMS, BH;
+    // A / A2 and S / S2 alternate with the step parity (read one, write the other)
+    int64_t A, A2, GA, S, S2, V, Bk, Hg, USUM, MU, Fv, Fsp, Gsp, KH, LUT, US, SUMS, BH;
     int64_t AL;        // ALPHA = |A - GA| for the divergence oscillator [C][N]
     int64_t KST, KSP;  // staging (second pool, commit only): kernel fields and spectra
     int nkf;           // staged fields: K kernels (+1 blur kernel)
@@ -38,6 +39,13 @@ struct DevWorld {
 // transport kernel takes the rest.
 __host__ __device__ inline bool sweep_world(const DevWorld &w) {
     return w.mode == 1 && w.dd == 1 && !(w.osc_la || w.osc_flow || w.osc_div_flow);
+}
+
+// Worlds moved by SRT without oscillators: each cell's transport reads only its own mu,
+// so the gradient sweep moves them and applies the end-of-step transform itself (no
+// transport launch, mu never written to HBM unless kept).
+__host__ __device__ inline bool srt_local_world(const DevWorld &w) {
+    return w.mode == 0 && !(w.osc_la || w.osc_flow || w.osc_div_flow);
 }
 
 // quirk bits (same values as lenia_hip.h)

@@ -59,6 +59,15 @@ __device__ __forceinline__ const float *A_cur(const PhaseArgs &a, const DevWorld
 __device__ __forceinline__ float *A_next(const PhaseArgs &a, const DevWorld &w) {
     return a.pool + (a.parity ? w.A : w.A2);
 }
+// S (the end-of-step sum the gradients read) alternates like A: a step reads S_cur and
+// its end-of-step transform writes S_next, so no cell's new S can reach a neighbour's
+// gradient within the step, whichever kernel runs the transform.
+__device__ __forceinline__ const float *S_cur(const PhaseArgs &a, const DevWorld &w) {
+    return a.pool + (a.parity ? w.S2 : w.S);
+}
+__device__ __forceinline__ float *S_next(const PhaseArgs &a, const DevWorld &w) {
+    return a.pool + (a.parity ? w.S : w.S2);
+}
 
 // exp for the analytic growth bell: Cody-Waite reduction x = n ln2 + r (fdlibm's split of
 // ln2 into a 33-bit head and a tail), the degree-13 Taylor polynomial of e^r in Horner
@@ -169,12 +178,18 @@ __device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w
 // returns the gradient of staged field f (0: S or SUM_S, 1 + c: H_c or U_S_c).  A
 // quirk-index alpha source outside the channels gives alpha = 0, which equals
 // clamp((0 / theta)^2, 0, 1).
+// For an SRT world moved here (srt_local_world) it also forms each channel's transport
+// term tt[c] from the cell's own mu (srt reintegration.cpp:607-688, as transport_cell
+// mode 0: the same expression on the same mu value), and writes mu only when kept.
 template <int W, int H, int D, int CM, class Grad>
 __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
-                                           const LocalIn<CM> &in, Grad grad) {
+                                           const LocalIn<CM> &in, Grad grad, float (&tt)[CM]) {
     using S = Shape<W, H, D>;
     const int C = w.C;
     const float *A = A_cur(a, w);
+    const bool srt = srt_local_world(w), store_mu = !srt || w.keep_mu;
+    const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
+    const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A : a.pool + w.Bk;
     const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
     const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
     const bool wall = w.border == 1;
@@ -225,13 +240,30 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
                 el(MU, (unsigned)((3 * c + d) * S::N) + iu) = clamp_med3(src * dt, nma, ma);  // -ma < 0 < ma
             }
         } else {
+            float mv[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 float m = (w.mu_vel ? v[d] : F[d]) * dt;
                 m = clamp_med3(m, nma, ma);  // -ma < 0 < ma
                 m = pos[d] + m;
                 if (wall) m = clamp_med3(m, sig, lim[d]);  // 0 < sigma < n - sigma
-                el(MU, (unsigned)((3 * c + d) * S::N) + iu) = m;
+                mv[d] = m;
+                if (store_mu) el(MU, (unsigned)((3 * c + d) * S::N) + iu) = m;
+            }
+            if (srt) {
+                float mm[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    float dv = pos[d] - mv[d];
+                    dv = dv < 0.0f ? -dv : dv;
+                    dv = w.si - dv;
+                    mm[d] = fmin_ref(fmax_ref(dv, 0.0f), w.clip);
+                }
+                float wt = el(ARR, (unsigned)(c * S::N) + iu) * mm[0];
+                wt = wt * mm[1];
+                wt = wt * mm[2];
+                wt = wt * w.sn;
+                tt[c] = c < nrt ? 0.0f + wt : 0.0f;  // TT starts at 0 (srt adds to it)
             }
         }
         if (!w.mu_vel && w.v_live) {  // (MU_VEL implies a live V)
@@ -409,8 +441,8 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
     const int C = w.C;
     const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
     const float *Acur = A_cur(a, w);
-    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
-    const float *Hg = a.pool + w.Hg;
+    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sn = S_next(a, w);
+    const float *Hg = a.pool + w.Hg, *Sg = S_cur(a, w);
     float ttsum = 0.0f;
 #pragma unroll
     for (int c = 0; c < LHD_MAX_C; c++) {
@@ -430,7 +462,55 @@ __device__ __forceinline__ void trm_cell(const PhaseArgs &a, const DevWorld &w, 
         An[g] = t;
     }
     const float sold = (w.quirks & LHQ_SUM_RESET) ? 0.0f : Sg[i];
-    Sg[i] = w.stam * sold + ttsum * w.stas;
+    Sn[i] = w.stam * sold + ttsum * w.stas;
+}
+
+// trm_cell split in two so the sweep can load a cell's inputs a step before it needs
+// them: trm_load reads them, trm_store applies the same arithmetic as trm_cell.
+// The growth-after-transport inputs (H, A) are read at use; GA and S are prefetched.
+template <int CM>
+struct TrmIn {
+    float ga[CM], s;
+};
+
+template <int W, int H, int D, int CM>
+__device__ __forceinline__ void trm_load(const PhaseArgs &a, const DevWorld &w, long long i, TrmIn<CM> &in) {
+    using S = Shape<W, H, D>;
+    const float *GA = a.pool + w.GA;
+#pragma unroll
+    for (int c = 0; c < CM; c++) in.ga[c] = c < w.C ? el(GA, (unsigned)(c * S::N + i)) : 0.0f;
+    in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : el(S_cur(a, w), (unsigned)i);
+}
+
+// C <= CM for every world this is used on.
+template <int W, int H, int D, int CM>
+__device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w, long long i, const float (&tt)[CM],
+                                          const TrmIn<CM> &in) {
+    using S = Shape<W, H, D>;
+    const int C = w.C;
+    const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
+    const float *Acur = A_cur(a, w), *Hg = a.pool + w.Hg;
+    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = S_next(a, w);
+    float ttsum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        if (c >= C) break;
+        const unsigned g = (unsigned)(c * S::N + i);
+        float t = tt[c];
+        if (w.growth_type == 2) {
+            const bool soft = (w.soft_mask >> c) & 1;
+            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) el(a.pool + w.Bk, g) = soft_growth(el(Acur, g), el(Hg, g), w.dt);
+            else if (soft) t = soft_growth(t, el(Hg, g), w.dt);
+            else t = clamp01 ? clamp_med3(t + el(Hg, g) * w.dt, 0.0f, 1.0f) : t + el(Hg, g) * w.dt;
+        } else if (clamp01) {
+            // TT starts at +0 and only gains products, so it (and TT + dt H) is never -0
+            t = clamp_med3(t, 0.0f, 1.0f);
+        }
+        ttsum = ttsum + t;
+        el(GA, g) = w.gtam * in.ga[c] + t * w.gtas;
+        el(An, g) = t;
+    }
+    el(Sg, (unsigned)i) = w.stam * in.s + ttsum * w.stas;
 }
 
 // Planes per sweep workgroup: the whole depth when the batch alone gives >= 2048
@@ -528,7 +608,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     // from padded index 5P/2 on
     const bool cdcopy = w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
     const bool blur_src = w.gradient == 0 && w.blur;
-    const float *srcS = a.pool + (blur_src ? w.SUMS : w.S);
+    const float *srcS = blur_src ? a.pool + w.SUMS : S_cur(a, w);
     const float *srcH = a.pool + (blur_src ? w.US : w.Hg);
     const int nitems = (C + 1) * T::SP;
     int fld[LI], xy[LI], pqxy[LI];
@@ -586,6 +666,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     fetch(z0 + 1);
     const int cx = tid % T::TX, cy = tid / T::TX;
     const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
+    const bool srt = srt_local_world(w);
     const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
     LocalIn<CM> in;
     local_load<W, H, D, CM>(a, w, col + (long long)W * H * z0, in);
@@ -598,6 +679,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
                     *p2 = ring + ((z + 2) % 3) * F * T::SP;
         const LocalIn<CM> cur = in;
         if (z + 1 < z1) local_load<W, H, D, CM>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
+        float tt[CM];
         local_cell<W, H, D, CM>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
             const float *const pl[3] = {p0 + f * T::SP, p1 + f * T::SP, p2 + f * T::SP};
             if (w.gradient == 1) {
@@ -606,56 +688,14 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
             } else {
                 cdiff_planes<T::SX>(pl, cx, cy, g);
             }
-        });
-    }
-}
-
-// trm_cell split in two so the sweep can load a cell's inputs a step before it needs
-// them: trm_load reads them, trm_store applies the same arithmetic as trm_cell.
-// The growth-after-transport inputs (H, A) are read at use; GA and S are prefetched.
-template <int CM>
-struct TrmIn {
-    float ga[CM], s;
-};
-
-template <int W, int H, int D, int CM>
-__device__ __forceinline__ void trm_load(const PhaseArgs &a, const DevWorld &w, long long i, TrmIn<CM> &in) {
-    using S = Shape<W, H, D>;
-    const float *GA = a.pool + w.GA;
-#pragma unroll
-    for (int c = 0; c < CM; c++) in.ga[c] = c < w.C ? el(GA, (unsigned)(c * S::N + i)) : 0.0f;
-    in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : el(a.pool + w.S, (unsigned)i);
-}
-
-// C <= CM for every world this is used on.
-template <int W, int H, int D, int CM>
-__device__ __forceinline__ void trm_store(const PhaseArgs &a, const DevWorld &w, long long i, const float (&tt)[CM],
-                                          const TrmIn<CM> &in) {
-    using S = Shape<W, H, D>;
-    const int C = w.C;
-    const bool clamp01 = (w.quirks & LHQ_CLAMP01) != 0;
-    const float *Acur = A_cur(a, w), *Hg = a.pool + w.Hg;
-    float *An = A_next(a, w), *GA = a.pool + w.GA, *Sg = a.pool + w.S;
-    float ttsum = 0.0f;
-#pragma unroll
-    for (int c = 0; c < CM; c++) {
-        if (c >= C) break;
-        const unsigned g = (unsigned)(c * S::N + i);
-        float t = tt[c];
-        if (w.growth_type == 2) {
-            const bool soft = (w.soft_mask >> c) & 1;
-            if (soft && (w.quirks & LHQ_SOFTCLIP_TR)) el(a.pool + w.Bk, g) = soft_growth(el(Acur, g), el(Hg, g), w.dt);
-            else if (soft) t = soft_growth(t, el(Hg, g), w.dt);
-            else t = clamp01 ? clamp_med3(t + el(Hg, g) * w.dt, 0.0f, 1.0f) : t + el(Hg, g) * w.dt;
-        } else if (clamp01) {
-            // TT starts at +0 and only gains products, so it (and TT + dt H) is never -0
-            t = clamp_med3(t, 0.0f, 1.0f);
+        }, tt);
+        if (srt) {  // the world's transport is this cell's own: the end-of-step transform here
+            const long long i = col + (long long)W * H * z;
+            TrmIn<CM> ti;
+            trm_load<W, H, D, CM>(a, w, i, ti);
+            trm_store<W, H, D, CM>(a, w, i, tt, ti);
         }
-        ttsum = ttsum + t;
-        el(GA, g) = w.gtam * in.ga[c] + t * w.gtas;
-        el(An, g) = t;
     }
-    el(Sg, (unsigned)i) = w.stam * in.s + ttsum * w.stas;
 }
 
 // ------------------------------------------------------------------------------------
@@ -846,6 +886,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int skip_s
     const int ty0 = (trest % (H / T::TY)) * T::TY, tz0 = (trest / (H / T::TY)) * T::TZ;
     const DevWorld &w = a.worlds[a.ids[wi]];
     if (skip_sweep && sweep_world(w)) return;  // moved by k_transport_sweep
+    if (srt_local_world(w)) return;            // moved by k_local_sweep
     const int tid = threadIdx.x, C = w.C;
     float tt[T::CPT][LHD_MAX_C];
 #pragma unroll

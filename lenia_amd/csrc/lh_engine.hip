@@ -214,7 +214,7 @@ struct Engine {
     MeshBufs mesh;
     int parity = 0;
     bool spectra_dirty = true;
-    int opt_keep_v = 0, opt_keep_usum = 0, opt_graph = 1, opt_profile = 0, opt_autotune = 1;
+    int opt_keep_v = 0, opt_keep_usum = 0, opt_keep_mu = 0, opt_graph = 1, opt_profile = 0, opt_autotune = 1;
     int opt_schedule = -1;  // LH_OPT_SCHEDULE: -1 = autotuner / defaults, else the forced schedule
     // captured step sequences, keyed by (starting parity, steps): one graph launch
     // advances every shape class by that many steps; least recently used evicted
@@ -300,7 +300,7 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "dft_iy", c.gops->iy, a, c.maxK);
             run(e, "dft_ix_growth", c.gops->ix, a, 0);
             run(e, "local", c.ops->local, a, local_arg(c));
-            run(e, "transport", c.ops->transport, a, transport_arg(c));
+            if (c.any_sweep || c.any_other) run(e, "transport", c.ops->transport, a, transport_arg(c));
             continue;
         }
         if (c.fused_growth) {
@@ -315,7 +315,8 @@ void enqueue_step(Engine &e, int parity) {
             run(e, "inv_blur", c.ops->inv_blur, a, 0);
         }
         run(e, "local", c.ops->local, a, local_arg(c));
-        run(e, "transport", c.ops->transport, a, transport_arg(c));
+        // (a class of SRT worlds only was moved by the gradient sweep)
+        if (c.any_sweep || c.any_other) run(e, "transport", c.ops->transport, a, transport_arg(c));
         PhaseArgs an = args_for(e, c, parity ^ 1);
         run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));
     }
@@ -353,6 +354,7 @@ DevWorld make_dev(const HostWorld &hw, const Engine &e) {
     d.growth_type = p.growth_type; d.mu_vel = p.mu_vel; d.adv_vel = p.adv_vel; d.blur = p.blur && p.gradient == 0;
     d.keep_v = e.opt_keep_v;
     d.keep_usum = e.opt_keep_usum;
+    d.keep_mu = e.opt_keep_mu;
     const bool osc_vf = p.osc_flow || p.osc_div_flow;
     d.v_live = p.mu_vel || (p.mode == LH_MODE_ADV && p.adv_vel) || (osc_vf && p.osc_vel) || e.opt_keep_v;
     d.need_f = osc_vf && !p.osc_vel;
@@ -425,7 +427,7 @@ int64_t field_off(const Engine &e, const HostWorld &w, int field) {
     switch (field) {
     case LH_F_A: return e.parity ? d.A2 : d.A;
     case LH_F_GA: return d.GA;
-    case LH_F_S: return d.S;
+    case LH_F_S: return e.parity ? d.S2 : d.S;
     case LH_F_V: return d.V;
     case LH_F_H: return d.Hg;
     case LH_F_B: return d.Bk;
@@ -675,6 +677,10 @@ lh_status lh_set_option(void *p, int option, int value) {
             return fail(LH_EINVAL, "LH_OPT_SCHEDULE: -1, or zseg_local | zseg_transport << 8 | tiled << 16 | growth << 17");
         e->opt_schedule = value;
         return LH_OK;
+    case LH_OPT_KEEP_MU:
+        if (e->committed) return fail(LH_ESTATE, "option must be set before lh_commit");
+        e->opt_keep_mu = value != 0;
+        return LH_OK;
     case LH_OPT_USE_GRAPH: e->opt_graph = value != 0; return LH_OK;
     case LH_OPT_PROFILE: e->opt_profile = value != 0; return LH_OK;
     }
@@ -753,6 +759,7 @@ lh_status lh_commit(void *p) {
         d.A2 = take(C * N);
         d.GA = take(C * N);
         d.S = take(N);
+        d.S2 = take(N);
         d.V = take(3 * C * N);
         d.Bk = take(C * N);
         d.Hg = take(C * N);
@@ -774,7 +781,8 @@ lh_status lh_commit(void *p) {
         c.maxK = std::max(c.maxK, K);
         c.any_blur |= d.blur != 0;
         c.any_osc |= (d.osc_la || d.osc_flow || d.osc_div_flow);
-        (sweep_world(d) ? c.any_sweep : c.any_other) = true;
+        if (sweep_world(d)) c.any_sweep = true;
+        else if (!srt_local_world(d)) c.any_other = true;  // (SRT worlds move in the gradient sweep)
     }
     e->pool_floats = (size_t)off;
     HIPCHK(hipMalloc(&e->pool, e->pool_floats * sizeof(float)));
@@ -855,6 +863,8 @@ lh_status lh_get_field(void *p, int world, int field, float *dst, size_t count) 
     HostWorld &hw = e->worlds[world];
     if (field < LH_F_A || field > LH_F_USUM) return fail(LH_EINVAL, "bad field");
     if (field == LH_F_USUM && !hw.dev.keep_usum) return fail(LH_ESTATE, "U_SUM needs LH_OPT_KEEP_USUM");
+    if (field == LH_F_MU && srt_local_world(hw.dev) && !hw.dev.keep_mu)
+        return fail(LH_ESTATE, "MU of an SRT world needs LH_OPT_KEEP_MU");
     if (count != field_count(hw, field)) return fail(LH_EINVAL, "count does not match the field size");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipMemcpyAsync(dst, e->pool + field_off(*e, hw, field), count * sizeof(float), hipMemcpyDeviceToHost,

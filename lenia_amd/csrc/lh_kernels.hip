@@ -54,7 +54,7 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
         out = reinterpret_cast<float2 *>(a.pool + w.Fsp) + (long long)f * S::Nh;
     } else if (src == SRC_HBLUR) {
         nf = w.C + 1;
-        in = f < w.C ? a.pool + w.Hg + (long long)f * S::N : a.pool + w.S;
+        in = f < w.C ? a.pool + w.Hg + (long long)f * S::N : S_cur(a, w);
         out = reinterpret_cast<float2 *>(a.pool + w.Fsp) + (long long)f * S::Nh;
     } else {
         nf = w.nkf;

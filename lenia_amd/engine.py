@@ -19,7 +19,7 @@ class Engine:
               "MU": L.F_MU, "USUM": L.F_USUM}
 
     def __init__(self, device: int = 0, keep_velocity=False, keep_usum=False, graph=True,
-                 profile=False, autotune=True, schedule=None):
+                 profile=False, autotune=True, schedule=None, keep_mu=False):
         self._L = L.load()
         h = C.c_void_p()
         L.check(self._L.lh_create(device, C.byref(h)), "lh_create")
@@ -27,6 +27,7 @@ class Engine:
         self.worlds = []
         L.check(self._L.lh_set_option(self._h, L.OPT_KEEP_VELOCITY, int(keep_velocity)), "keep_velocity")
         L.check(self._L.lh_set_option(self._h, L.OPT_KEEP_USUM, int(keep_usum)), "keep_usum")
+        L.check(self._L.lh_set_option(self._h, L.OPT_KEEP_MU, int(keep_mu)), "keep_mu")
         self.set_option(L.OPT_USE_GRAPH, graph)
         self.set_option(L.OPT_PROFILE, profile)
         self.set_option(L.OPT_AUTOTUNE, autotune)

@@ -15,7 +15,8 @@ def run_pair(world: dict, steps: int, fields=("A",), keep_velocity=True, keep_us
     ow.set(O.F_A, world["A"])
     if "V" in world:
         ow.set(O.F_V, world["V"])
-    eng = Engine(keep_velocity=keep_velocity, keep_usum=keep_usum, graph=graph, schedule=schedule)
+    eng = Engine(keep_velocity=keep_velocity, keep_usum=keep_usum, graph=graph, schedule=schedule,
+                 keep_mu="MU" in fields)
     wid = eng.add_world(world["shape"], world["C"], world["kernels"], world["cfg"])
     eng.commit()
     eng.set(wid, "A", world["A"])
