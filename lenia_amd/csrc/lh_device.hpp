@@ -181,13 +181,14 @@ __device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w
 // For an SRT world moved here (srt_local_world) it also forms each channel's transport
 // term tt[c] from the cell's own mu (srt reintegration.cpp:607-688, as transport_cell
 // mode 0: the same expression on the same mu value), and writes mu only when kept.
-template <int W, int H, int D, int CM, class Grad>
+// SRTL: compiled with that path (a launch whose shape class holds such worlds).
+template <int W, int H, int D, int CM, bool SRTL, class Grad>
 __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
                                            const LocalIn<CM> &in, Grad grad, float (&tt)[CM]) {
     using S = Shape<W, H, D>;
     const int C = w.C;
     const float *A = A_cur(a, w);
-    const bool srt = srt_local_world(w), store_mu = !srt || w.keep_mu;
+    const bool srt = SRTL && srt_local_world(w), store_mu = !srt || w.keep_mu;
     const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
     const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A : a.pool + w.Bk;
     const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
@@ -588,7 +589,7 @@ __device__ __forceinline__ void cdiff_planes(const float *const (&pl)[3], int tx
 // HBM once per column instead of once per tile it borders.
 // zseg: planes per workgroup (the depth is split into segments when few worlds would
 // leave the GPU idle).
-template <int W, int H, int D, int CM>
+template <int W, int H, int D, int CM, bool SRTL>
 __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_sweep(PhaseArgs a, int zseg) {
     using S = Shape<W, H, D>;
     using T = SweepCfg<W, H, D, CM>;
@@ -666,7 +667,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     fetch(z0 + 1);
     const int cx = tid % T::TX, cy = tid / T::TX;
     const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
-    const bool srt = srt_local_world(w);
+    const bool srt = SRTL && srt_local_world(w);
     const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
     LocalIn<CM> in;
     local_load<W, H, D, CM>(a, w, col + (long long)W * H * z0, in);
@@ -680,7 +681,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
         const LocalIn<CM> cur = in;
         if (z + 1 < z1) local_load<W, H, D, CM>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
         float tt[CM];
-        local_cell<W, H, D, CM>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
+        local_cell<W, H, D, CM, SRTL>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
             const float *const pl[3] = {p0 + f * T::SP, p1 + f * T::SP, p2 + f * T::SP};
             if (w.gradient == 1) {
                 if (perm) sobel_planes<T::SX, true>(pl, cx, cy, g);
@@ -696,6 +697,24 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
             trm_store<W, H, D, CM>(a, w, i, tt, ti);
         }
     }
+}
+
+// Launches the gradient sweep of a shape class.  arg: bit 0 the class holds SRT worlds
+// moved inside the sweep, maxC << 8, planes per workgroup << 16 (0: sweep_zseg).
+template <int W, int H, int D, int CM, bool SRTL>
+void launch_local_sweep_cm(const PhaseArgs &a, int arg, hipStream_t st) {
+    using SC = SweepCfg<W, H, D, CM>;
+    const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+    hipLaunchKernelGGL((k_local_sweep<W, H, D, CM, SRTL>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
+                       st, a, zs);
+}
+template <int W, int H, int D>
+void launch_local_sweep(const PhaseArgs &a, int arg, hipStream_t st) {
+    const bool c3 = ((arg >> 8) & 0xff) <= 3, srt = arg & 1;
+    if (c3 && srt) launch_local_sweep_cm<W, H, D, 3, true>(a, arg, st);
+    else if (c3) launch_local_sweep_cm<W, H, D, 3, false>(a, arg, st);
+    else if (srt) launch_local_sweep_cm<W, H, D, LHD_MAX_C, true>(a, arg, st);
+    else launch_local_sweep_cm<W, H, D, LHD_MAX_C, false>(a, arg, st);
 }
 
 // ------------------------------------------------------------------------------------

@@ -168,6 +168,7 @@ struct ShapeClass {
     bool any_blur = false;
     bool any_osc = false;
     bool any_sweep = false, any_other = false;  // transport by the z-sweep / by tiles
+    bool any_srt_local = false;                 // SRT worlds moved by the gradient sweep
     // schedule (lh_commit's autotuner; 0 / false = the launchers' defaults)
     int zseg_local = 0, zseg_transport = 0;
     bool tiled_transport = false;  // the tiled kernel also moves the sweep's worlds
@@ -279,7 +280,7 @@ int transport_arg(const ShapeClass &c) {
     return (c.any_osc ? 1 : 0) | (sweep ? 2 : 0) | (tiles ? 4 : 0) | (c.maxC << 8) | (c.zseg_transport << 16);
 }
 
-int local_arg(const ShapeClass &c) { return (c.maxC << 8) | (c.zseg_local << 16); }
+int local_arg(const ShapeClass &c) { return (c.any_srt_local ? 1 : 0) | (c.maxC << 8) | (c.zseg_local << 16); }
 
 void enqueue_spectra(Engine &e) {
     for (ShapeClass &c : e.classes) {
@@ -782,7 +783,8 @@ lh_status lh_commit(void *p) {
         c.any_blur |= d.blur != 0;
         c.any_osc |= (d.osc_la || d.osc_flow || d.osc_div_flow);
         if (sweep_world(d)) c.any_sweep = true;
-        else if (!srt_local_world(d)) c.any_other = true;  // (SRT worlds move in the gradient sweep)
+        else if (srt_local_world(d)) c.any_srt_local = true;
+        else c.any_other = true;
     }
     e->pool_floats = (size_t)off;
     HIPCHK(hipMalloc(&e->pool, e->pool_floats * sizeof(float)));

@@ -34,10 +34,15 @@ __device__ __forceinline__ float2 c_mul(float2 a, float2 w) {
     return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
 }
 
+// Twiddle tables are read-only for a kernel's lifetime: read through the constant
+// address space, so a wave-uniform index (every register-resident transform's) becomes a
+// scalar load instead of a vector load per lane.
+typedef const __attribute__((address_space(4))) float *tw_const_ptr;
 template <bool INV>
 __device__ __forceinline__ float2 twiddle(const float2 *__restrict__ tw, int e) {
-    float2 t = tw[e];
-    return INV ? t : make_float2(t.x, -t.y);
+    const tw_const_ptr c = (tw_const_ptr)tw;
+    const float re = c[2 * e], im = c[2 * e + 1];
+    return make_float2(re, INV ? im : -im);
 }
 
 template <int R, int N, bool INV>

@@ -721,20 +721,8 @@ struct Launch {
             else growth_fused_cm<LHD_MAX_C, true>(a, st);
         }
     }
-    // arg: maxC << 8 | planes per sweep workgroup << 16 (0: sweep_zseg)
-    static void local(const PhaseArgs &a, int arg, void *st) {
-        if (((arg >> 8) & 0xff) <= 3) {
-            using SC = SweepCfg<W, H, D, 3>;
-            const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
-            hipLaunchKernelGGL((k_local_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
-                               (hipStream_t)st, a, zs);
-        } else {
-            using SC = SweepCfg<W, H, D, LHD_MAX_C>;
-            const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
-            hipLaunchKernelGGL((k_local_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
-                               dim3(SC::THREADS), 0, (hipStream_t)st, a, zs);
-        }
-    }
+    // arg: bit 0 SRT worlds moved in the sweep | maxC << 8 | planes per sweep workgroup << 16
+    static void local(const PhaseArgs &a, int arg, void *st) { launch_local_sweep<W, H, D>(a, arg, (hipStream_t)st); }
     // arg bits: 1 some world enables an oscillator, 2 some world is moved by the
     // z-sweep (RT, dd = 1, no oscillator), 4 some world is not (or the tiled kernel moves
     // every world); maxC << 8; planes per sweep workgroup << 16 (0: sweep_zseg)
