@@ -314,7 +314,11 @@ lh_status lh_profile_reset(void *engine);
  * kernel moving `bytes` bytes (rounded down to 16) on the engine stream, synchronous;
  * its FETCH_SIZE / WRITE_SIZE counters calibrate HBM-traffic readings. */
 lh_status lh_calibration_copy(void *engine, size_t bytes);
-/* Whether a shape is supported by this build. */
+/* Whether a shape is supported: 1 built into this library, 2 as a shape module
+ * (liblenia_shape_<W>x<H>x<D>.so in $LENIA_SHAPE_DIR or in shapes/ beside the library,
+ * built by `python -m lenia_amd.build --shape WxHxD` for any sides divisible by 4 from 4
+ * to 128, as the reference allows any such GRID_WIDTH x GRID_HEIGHT, render.h:27-29,
+ * README.md:30), 0 neither.  Host only: no HIP call, no module loaded. */
 int lh_shape_supported(int W, int H, int D);
 const char *lh_last_error(void);
 const char *lh_version(void);

@@ -90,7 +90,11 @@ struct ShapeOps {
     launch_fn growth_fused; // spec_z + inv_growth in one launch, or nullptr (spectrum too big for LDS)
 };
 
+// The launchers of a shape: built into the library, or from a shape module
+// (liblenia_shape_<W>x<H>x<D>.so) loaded on first use; nullptr when neither exists.
 const ShapeOps *lh_shape_ops(int W, int H, int D);
+bool lh_shape_builtin(int W, int H, int D);
+bool lh_shape_module_exists(int W, int H, int D);
 
 // One world to seed (lh_seed_random): float offsets of its current A, GA, S, V in the
 // pool, and its extents.

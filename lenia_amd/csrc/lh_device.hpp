@@ -124,9 +124,20 @@ __device__ __forceinline__ float soft_growth(float g, float h, float dt) {
 // cells (whole x rows) and stages the block plus a one-cell halo in LDS, with every
 // halo value resolved once at load time, so the per-cell loops carry no branches.
 // ------------------------------------------------------------------------------------
+// x-width of a stencil kernel's tile: whole rows up to 48 cells; wider rows in tiles of
+// 32 cells where the width is a multiple of 32, else in the fewest equal tiles of at most
+// 48 cells.
+constexpr int stencil_tx(int W) {
+    if (W <= 48) return W;
+    if (W % 32 == 0) return 32;
+    for (int n = (W + 47) / 48; n <= W; n++)
+        if (W % n == 0) return W / n;
+    return 1;
+}
+
 template <int W, int H, int D>
 struct TileCfg {
-    static constexpr int TX = W <= 48 ? W : 32;  // whole rows up to 48, else 32-wide x tiles
+    static constexpr int TX = stencil_tx(W);
     static constexpr int XT = W / TX;
     static constexpr int TY = H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1);
     static constexpr int TZ = D % 4 == 0 ? 4 : (D % 2 == 0 ? 2 : 1);
@@ -527,7 +538,7 @@ inline int sweep_zseg(int nworlds, int blocks_per_world, int depth) {
 // a TX x TY column of cells through the whole depth and stages one z-plane at a time.
 template <int W, int H, int D, int CM, int TYMAX = 8>
 struct SweepCfg {
-    static constexpr int TX = W <= 48 ? W : 32;
+    static constexpr int TX = stencil_tx(W);
     static constexpr int XT = W / TX;
     // 8 rows when the plane's weights for 4 channels fit 64 KB of LDS, else 4
     static constexpr int TY = (TYMAX >= 8 && TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));

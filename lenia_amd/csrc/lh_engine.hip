@@ -326,7 +326,9 @@ void enqueue_step(Engine &e, int parity) {
 lh_status validate(const lh_shape &s, const lh_params &p, const lh_kernel *k) {
     if (!lh_shape_ops(s.W, s.H, s.D))
         return fail(LH_EUNSUPPORTED, "shape " + std::to_string(s.W) + "x" + std::to_string(s.H) + "x" +
-                                         std::to_string(s.D) + " is not built into this library");
+                                         std::to_string(s.D) + " is neither built into this library nor available "
+                                         "as a shape module (python -m lenia_amd.build --shape " +
+                                         std::to_string(s.W) + "x" + std::to_string(s.H) + "x" + std::to_string(s.D) + ")");
     if (s.C < 1 || s.C > LH_MAX_C) return fail(LH_EINVAL, "channel count must be 1..4");
     if (s.K < 0 || s.K > LH_MAX_K) return fail(LH_EINVAL, "kernel count must be 0..16");
     if (s.K > 0 && !k) return fail(LH_EINVAL, "kernels pointer is null");
@@ -1407,7 +1409,10 @@ lh_status lh_calibration_copy(void *p, size_t bytes) {
     return LH_OK;
 }
 
-int lh_shape_supported(int W, int H, int D) { return lh_shape_ops(W, H, D) != nullptr; }
+int lh_shape_supported(int W, int H, int D) {
+    if (lh_shape_builtin(W, H, D)) return 1;
+    return lh_shape_module_exists(W, H, D) ? 2 : 0;
+}
 
 const char *lh_last_error(void) { return g_err.c_str(); }
 

@@ -262,8 +262,10 @@ class Engine:
         L.check(self._L.lh_profile_reset(self._h), "lh_profile_reset")
 
 
-def shape_supported(W, H, D) -> bool:
-    return bool(L.load().lh_shape_supported(W, H, D))
+def shape_supported(W, H, D) -> int:
+    """1: built into the library; 2: available as a shape module (build one with
+    `python -m lenia_amd.build --shape WxHxD`); 0: neither."""
+    return int(L.load().lh_shape_supported(W, H, D))
 
 
 VERTEX_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("nx", "<f4"), ("ny", "<f4"),

@@ -51,7 +51,7 @@ static inline cpx cmul(cpx a, cpx w) {
 /* once to REAL.  Odd radices use a direct DFT accumulated in index order.          */
 /* ------------------------------------------------------------------------------ */
 #define MAXN 256
-#define MAXR 16
+#define MAXR 32
 typedef struct { int n, nst, radix[16]; real c[MAXN], s[MAXN]; } fplan;
 static fplan *g_plans[MAXN + 1];
 

@@ -45,3 +45,26 @@ def test_shape_table_is_host_only():
     assert L.lh_shape_supported(44, 24, 44) == 1
     assert L.lh_shape_supported(30, 30, 30) == 0
     assert L.lh_version().startswith(b"lenia_hip")
+
+
+def test_shape_modules_found_without_loading(tmp_path, monkeypatch):
+    """A shape outside the built-in set is reported as a module (2) when its file is in
+    $LENIA_SHAPE_DIR or shapes/ beside the library; nothing is loaded to answer."""
+    from lenia_amd import _lib, build as B
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("liblenia_hip.so not built")
+    L = _lib.load()
+    for shape in B.TEST_SHAPES:
+        if os.path.exists(B.shape_module_path(*shape)):
+            assert L.lh_shape_supported(*shape) == 2, shape
+    assert L.lh_shape_supported(60, 8, 60) == 0
+    (tmp_path / "liblenia_shape_60x8x60.so").write_bytes(b"")
+    monkeypatch.setenv("LENIA_SHAPE_DIR", str(tmp_path))
+    assert L.lh_shape_supported(60, 8, 60) == 2
+
+
+def test_shape_module_sides_checked():
+    from lenia_amd import build as B
+    for bad in ((30, 32, 32), (32, 0, 32), (132, 32, 32), (32, 32, 2)):
+        with pytest.raises(ValueError):
+            B._shape_module_cmd(*bad, "/dev/null")

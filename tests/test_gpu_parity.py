@@ -140,6 +140,37 @@ def test_large_worlds_bitwise(shape, C, K, R, steps, over):
         assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
 
 
+@pytest.mark.parametrize("shape,C,K,R,steps,over", [
+    # radix 7, register-resident transforms; every phase, growth fused where it fits
+    ((28, 28, 28), 3, 9, 6, 3, dict(asymptotic=ASYM)),
+    ((28, 28, 28), 2, 4, 5, 2, dict(mode="srt", border="wall", sobel_border="wall", mu_vel=True)),
+    # non-cube with radices 5 and 3
+    ((20, 12, 20), 3, 6, 4, 3, dict(gradient="central", blur=True, sobel_border="none", border="none",
+                                    asymptotic=ASYM)),
+    # above 48: LDS-staged transforms with radix 13, stencil tiles of 26 cells
+    ((52, 52, 52), 2, 4, 7, 2, dict(mode="adv", asymptotic=ASYM)),
+    ((52, 52, 52), 3, 5, 7, 2, dict(asymptotic=ASYM)),
+])
+def test_shape_modules_bitwise(shape, C, K, R, steps, over):
+    """Shapes outside the built-in set run from shape modules built on demand
+    (liblenia_shape_<W>x<H>x<D>.so), bit-identical to the checker like the built-in ones."""
+    import os
+    from lenia_amd.engine import shape_supported
+    from oracle import oracle as O
+    assert shape_supported(*shape) == 2, f"{shape}: shape module not built (lenia_amd.build.TEST_SHAPES)"
+    w = WL.world(shape, C, K, R, seed=sum(shape) + steps + C, **over)
+    fields = ["A", "H", "S"] + (["MU"] if w["cfg"]["mode"] != "adv" else []) + \
+        (["V"] if w["cfg"]["mode"] == "adv" or w["cfg"].get("mu_vel") else [])
+    O.set_threads(min(8, os.cpu_count() or 1))
+    try:
+        r = run_pair(w, steps, fields=fields, keep_velocity=True)
+    finally:
+        O.set_threads(1)
+    for f in fields:
+        bad, err = mismatch(*r[f])
+        assert bad == 0, f"{shape}: {f} differs in {bad} values (max abs {err:.3e})"
+
+
 @pytest.mark.parametrize("shape,C,K,R", [((16, 8, 16), 3, 5, 3), ((64, 64, 64), 2, 4, 6)])
 def test_oscillators_shapes_bitwise(shape, C, K, R):
     w = WL.world(shape, C, K, R, seed=sum(shape) + 7, asymptotic=ASYM, mu_vel=True, **OSC, osc_vel=1)
