@@ -355,6 +355,17 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     }
     float *Hg = a.pool + w.Hg + (long long)z * PL, *B = a.pool + w.Bk + (long long)z * PL;
     float *US = a.pool + w.USUM + (long long)z * PL;
+    // A for B loaded before any store (loads and stores share one completion counter: a
+    // load issued after a store waits for it)
+    const bool with_b = w.growth_type == 1;
+    float av[CPT][CM];
+    if (with_b) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++)
+#pragma unroll
+            for (int c = 0; c < CM; c++)
+                av[j][c] = (tid + j * T < PLS && c < C) ? A[(long long)c * S::N + c0 + tid + j * T] : 0.0f;
+    }
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
         const int i = c0 + tid + j * T;
@@ -366,10 +377,8 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
             const float hv = h[j][c];
             Hg[g] = hv;
             if (KU) US[g] = us[j][c];
-            if (w.growth_type == 1) {
-                const float av = A[g];
-                B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(av, hv, w.dt) : clamp_ref(av + hv * w.dt, 0.0f, 1.0f);
-            }
+            if (with_b)
+                B[g] = ((w.soft_mask >> c) & 1) ? soft_growth(av[j][c], hv, w.dt) : clamp_ref(av[j][c] + hv * w.dt, 0.0f, 1.0f);
         }
     }
 }
@@ -566,15 +575,28 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
             for (int e = 0; e < 2 * W; e++) stage[z * F::ZP + y * W + e] = pass == 0 ? h[j][e] : us[j][KU ? e : 0];
         }
         __syncthreads();
-        for (int i = tid; i < S::N; i += T) {
+        // this thread's A values are loaded first, all in flight before any store: loads
+        // and stores share one completion counter, so a load issued after a store would
+        // wait for that store too (one memory round trip per cell instead of one in all)
+        constexpr int PER = (S::N + T - 1) / T;
+        const bool with_b = pass == 0 && growth_type == 1;
+        float av[PER];
+        if (with_b) {
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                const int i = tid + j * T;
+                av[j] = i < S::N ? Ac[i] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int i = tid + j * T;
+            if (i >= S::N) break;
             const int z = i / S::PL, r = i % S::PL;
             const float v = stage[z * F::ZP + r];
             if (pass == 0) {
                 Hg[i] = v;
-                if (growth_type == 1) {
-                    const float av = Ac[i];
-                    B[i] = soft ? soft_growth(av, v, dt) : clamp_ref(av + v * dt, 0.0f, 1.0f);
-                }
+                if (with_b) B[i] = soft ? soft_growth(av[j], v, dt) : clamp_ref(av[j] + v * dt, 0.0f, 1.0f);
             } else {
                 US[i] = v;
             }
