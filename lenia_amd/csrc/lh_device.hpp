@@ -683,10 +683,10 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     LocalIn<CM> in;
     local_load<W, H, D, CM>(a, w, col + (long long)W * H * z0, in);
     for (int z = z0; z < z1; z++) {
-        __syncthreads();  // the plane slot about to be refilled is no longer read
+        lds_barrier();  // the plane slot about to be refilled is no longer read
         store(z + 1);
         if (z + 2 <= z1) fetch(z + 2);
-        __syncthreads();
+        lds_barrier();
         const float *p0 = ring + (z % 3) * F * T::SP, *p1 = ring + ((z + 1) % 3) * F * T::SP,
                     *p2 = ring + ((z + 2) % 3) * F * T::SP;
         const LocalIn<CM> cur = in;
@@ -818,7 +818,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
     TrmIn<CM> pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
     auto step = [&](int zp, const float (&cur)[T::ITEMS][4], float (&nxt)[T::ITEMS][4]) {
         if (zp < z1) fetch(zp + 1, nxt);
-        __syncthreads();  // the previous plane's weights are no longer read
+        lds_barrier();  // the previous plane's weights are no longer read
         const float shz = !torus ? 0.0f : (zp < 0 ? (float)D : (zp >= D ? -(float)D : 0.0f));
         const float pz[3] = {(float)(zp + 1) + 0.5f, (float)zp + 0.5f, (float)(zp - 1) + 0.5f};
 #pragma unroll
@@ -846,7 +846,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int c = 0; c < CM; c++) {
             if (c >= nrt) break;
@@ -967,7 +967,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int skip_s
 #pragma unroll
         for (int c = 0; c < LHD_MAX_C; c++) {
             if (c >= nrt) break;
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int j = 0; j < SPT; j++) {
                 const int q = tid + j * T::THREADS;
@@ -988,7 +988,7 @@ __global__ __launch_bounds__(256) void k_transport_tiled(PhaseArgs a, int skip_s
                     }
                 }
             }
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int j = 0; j < T::CPT; j++) {
                 const int cell = tid + j * T::THREADS;

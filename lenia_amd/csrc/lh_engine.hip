@@ -324,11 +324,12 @@ void enqueue_step(Engine &e, int parity) {
 }
 
 lh_status validate(const lh_shape &s, const lh_params &p, const lh_kernel *k) {
+    const std::string dims = std::to_string(s.W) + "x" + std::to_string(s.H) + "x" + std::to_string(s.D);
+    if (s.W % 4 || s.H % 4 || s.D % 4 || s.W < 4 || s.H < 4 || s.D < 4 || s.W > 128 || s.H > 128 || s.D > 128)
+        return fail(LH_EUNSUPPORTED, "shape " + dims + " is not built: sides must be divisible by 4, 4..128");
     if (!lh_shape_ops(s.W, s.H, s.D))
-        return fail(LH_EUNSUPPORTED, "shape " + std::to_string(s.W) + "x" + std::to_string(s.H) + "x" +
-                                         std::to_string(s.D) + " is neither built into this library nor available "
-                                         "as a shape module (python -m lenia_amd.build --shape " +
-                                         std::to_string(s.W) + "x" + std::to_string(s.H) + "x" + std::to_string(s.D) + ")");
+        return fail(LH_EUNSUPPORTED, "shape " + dims + " is not built into this library and has no shape module "
+                                     "(python -m lenia_amd.build --shape " + dims + ")");
     if (s.C < 1 || s.C > LH_MAX_C) return fail(LH_EINVAL, "channel count must be 1..4");
     if (s.K < 0 || s.K > LH_MAX_K) return fail(LH_EINVAL, "kernel count must be 0..16");
     if (s.K > 0 && !k) return fail(LH_EINVAL, "kernels pointer is null");

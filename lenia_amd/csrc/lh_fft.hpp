@@ -10,6 +10,18 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Workgroup barrier ordering LDS only: this wave's LDS operations complete, then
+// s_barrier.  __syncthreads() fences global memory as well, which waits for every
+// outstanding global load and store of the wave -- draining a sweep's prefetched plane and
+// its stores at every step.  Used where a workgroup's threads share data through LDS only
+// (no thread reads global memory another thread of the launch writes, and no
+// global-to-LDS direct loads are in flight across the barrier).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int fft_nst(int n) {
     int m = n, st = 0;
     while (m % 4 == 0) { m /= 4; st++; }
@@ -139,7 +151,7 @@ __device__ __forceinline__ void fft_lds_stages(float2 *src, float2 *dst, const f
 #pragma unroll
             for (int r = 0; r < R; r++) dst[l * LDA + base + r * NS] = v[r];
         }
-        __syncthreads();
+        lds_barrier();
         fft_lds_stages<N, INV, LINES, LDA, NT, S + 1, NS * R>(dst, src, tw);
     }
 }
@@ -152,6 +164,6 @@ __device__ __forceinline__ void fft_lds_batch(float2 *a, float2 *b, const float2
             const int l = t % LINES, i = t / LINES;
             a[l * LDA + i] = b[l * LDA + i];
         }
-        __syncthreads();
+        lds_barrier();
     }
 }

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(128) void k_fwd_xy(PhaseArgs a, int arg) {
             o1[k] = make_float2((Z.y + P.y) * 0.5f, (P.x - Z.x) * 0.5f);
         }
     }
-    __syncthreads();
+    lds_barrier();
     for (int t = tid; t < NP * Wh; t += Cfg::THREADS) {
         const int p = t / Wh, kx = t % Wh;
         float2 line[H];
@@ -292,9 +292,9 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
     float *sgf = reinterpret_cast<float *>(sg);
     for (int k0 = 0; k0 < K; k0 += KB) {
         const int nk = K - k0 < KB ? K - k0 : KB;
-        __syncthreads();
+        lds_barrier();
         dma_planes<Cfg::PLANE, T>(sg, G + (long long)k0 * S::Nh + (long long)z * Cfg::PLANE, S::Nh, nk, tid);
-        __syncthreads();
+        __syncthreads();  // (global memory: the direct-to-LDS loads have landed)
         for (int t = tid; t < nk * Wh; t += T) {
             const int kk = t / Wh, kx = t % Wh;
             float2 line[H];
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
 #pragma unroll
             for (int y = 0; y < H; y++) p[y * Wh] = line[y];
         }
-        __syncthreads();
+        lds_barrier();
         for (int t = tid; t < nk * HP; t += T) {
             const int kk = t / HP, y = 2 * (t % HP) + part * (H / SPL);
             const float2 *ra = sg + kk * Cfg::PLANE + y * Wh, *rb = ra + Wh;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256, 4) void k_inv_growth(PhaseArgs a, int) {
 #pragma unroll
             for (int x = 0; x < W; x++) { u0[x] = line[x].x; u1[x] = line[x].y; }
         }
-        __syncthreads();
+        lds_barrier();
         // growth, mt_fft_step lenia.cpp:501-538: per cell and output channel, kernels in order
         for (int kk = 0; kk < nk; kk++) {
             const DevKernel &kp = w.kern[k0 + kk];
@@ -460,7 +460,7 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
         const DevKernel kp = wp->kern[k];
         if (kp.c1 != c1) continue;
         const float *oct = a.pool + kh_off + (long long)k * S::Noct;
-        __syncthreads();  // the previous kernel's spectrum (and the table) are no longer read
+        lds_barrier();  // the previous kernel's spectrum (and the table) are no longer read
         // z-columns: forward z-FFT of the source channel, x kernel octant, inverse z-FFT
         for (int col = tid; col < F::COLS; col += T) {
             const int kx = col % Wh, ky = col / Wh;
@@ -480,7 +480,7 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
 #pragma unroll
             for (int z = 0; z < D; z++) spec[z * PP + col] = t[z];
         }
-        __syncthreads();
+        lds_barrier();
         // y-lines: inverse y-FFT in place
         for (int ln = tid; ln < F::YL; ln += T) {
             const int kx = ln % Wh, z = ln / Wh;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
 #pragma unroll
             for (int y = 0; y < H; y++) p[y * Wh] = line[y];
         }
-        __syncthreads();
+        lds_barrier();
         // row pairs (z fastest): c2r along x, then the growth of the pair's 2W cells,
         // mt_fft_step lenia.cpp:501-538
 #pragma unroll
@@ -565,7 +565,7 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
     float *stage = reinterpret_cast<float *>(fused_lds);
 #pragma unroll
     for (int pass = 0; pass < (KU ? 2 : 1); pass++) {
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             const int rp = tid + j * T;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__((FusedCfg<W, H, D>::threads(KU)), 1) void k_growth_
 #pragma unroll
             for (int e = 0; e < 2 * W; e++) stage[z * F::ZP + y * W + e] = pass == 0 ? h[j][e] : us[j][KU ? e : 0];
         }
-        __syncthreads();
+        lds_barrier();
         // this thread's A values are loaded first, all in flight before any store: loads
         // and stores share one completion counter, so a load issued after a store would
         // wait for that store too (one memory round trip per cell instead of one in all)
@@ -619,9 +619,9 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
     float *sgf = reinterpret_cast<float *>(sg);
     for (int f0 = 0; f0 < NF; f0 += KB) {
         const int nk = NF - f0 < KB ? NF - f0 : KB;
-        __syncthreads();
+        lds_barrier();
         dma_planes<Cfg::PLANE, T>(sg, G + (long long)f0 * S::Nh + (long long)z * Cfg::PLANE, S::Nh, nk, tid);
-        __syncthreads();
+        __syncthreads();  // (global memory: the direct-to-LDS loads have landed)
         for (int t = tid; t < nk * Wh; t += T) {
             const int kk = t / Wh, kx = t % Wh;
             float2 line[H];
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
 #pragma unroll
             for (int y = 0; y < H; y++) p[y * Wh] = line[y];
         }
-        __syncthreads();
+        lds_barrier();
         for (int t = tid; t < nk * (H / 2); t += T) {
             const int kk = t / (H / 2), y = 2 * (t % (H / 2));
             const float2 *ra = sg + kk * Cfg::PLANE + y * Wh, *rb = ra + Wh;
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(256) void k_inv_blur(PhaseArgs a, int) {
 #pragma unroll
             for (int x = 0; x < W; x++) { u0[x] = line[x].x; u1[x] = line[x].y; }
         }
-        __syncthreads();
+        lds_barrier();
         for (int i = tid; i < nk * PL; i += T) {
             const int kk = i / PL, j = i % PL, y = j / W, x = j % W;
             const int f = f0 + kk;

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_fwd_xy_big(PhaseArgs a, int arg) {
         const int l = t / W, x = t % W;
         ba[l * LX + x] = make_float2(pl[(2 * l) * W + x], pl[(2 * l + 1) * W + x]);
     }
-    __syncthreads();
+    lds_barrier();
     fft_lds_batch<W, false, H / 2, LX, NT>(ba, bb, a.twW);
     // unpack the two real spectra into [kx][y] lines for the y transform
     for (int t = threadIdx.x; t < (H / 2) * Wh; t += NT) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_fwd_xy_big(PhaseArgs a, int arg) {
         bb[k * LY + 2 * l] = make_float2((Z.x + P.x) * 0.5f, (Z.y - P.y) * 0.5f);
         bb[k * LY + 2 * l + 1] = make_float2((Z.y + P.y) * 0.5f, (P.x - Z.x) * 0.5f);
     }
-    __syncthreads();
+    lds_barrier();
     fft_lds_batch<H, false, Wh, LY, NT>(bb, ba, a.twH);
     float2 *o = out + (long long)z * H * Wh;
     for (int t = threadIdx.x; t < H * Wh; t += NT) {
@@ -103,12 +103,12 @@ __global__ __launch_bounds__(256) void k_spec_z_big(PhaseArgs a, int mode) {
             for (int k = 0; k < w.K; k++) used |= (w.kern[k].c0 == c);
             if (!used) continue;
         }
-        __syncthreads();
+        lds_barrier();
         for (int t = threadIdx.x; t < CB * D; t += NT) {
             const int z = t / CB, cb = t % CB, col = col0 + cb;
             sT[cb * LZ + z] = col < COLS ? Fsp[(long long)c * S::Nh + (long long)z * COLS + col] : make_float2(0.0f, 0.0f);
         }
-        __syncthreads();
+        lds_barrier();
         fft_lds_batch<D, false, CB, LZ, NT>(sT, sS, a.twD);
         if (mode == 2) {
             float *oct = c < w.K ? a.pool + w.KH + (long long)c * S::Noct : a.pool + w.BH;
@@ -132,14 +132,14 @@ __global__ __launch_bounds__(256) void k_spec_z_big(PhaseArgs a, int mode) {
                 const float2 tv = sT[cb * LZ + z];
                 sY[cb * LZ + z] = make_float2(tv.x * kv, tv.y * kv);
             }
-            __syncthreads();
+            lds_barrier();
             fft_lds_batch<D, true, CB, LZ, NT>(sY, sS, a.twD);
             float2 *dst = Gsp + (long long)(mode == 0 ? k : c) * S::Nh;
             for (int t = threadIdx.x; t < CB * D; t += NT) {
                 const int z = t / CB, cb = t % CB, col = col0 + cb;
                 if (col < COLS) dst[(long long)z * COLS + col] = sY[cb * LZ + z];
             }
-            __syncthreads();
+            lds_barrier();
         }
     }
 }
@@ -156,7 +156,7 @@ __device__ __forceinline__ void inv_plane(const float2 *g, float2 *cols, float2 
         const int y = t / Wh, k = t % Wh;
         cols[k * LY + y] = g[y * Wh + k];
     }
-    __syncthreads();
+    lds_barrier();
     fft_lds_batch<H, true, Wh, LY, NT>(cols, lines, twH);
     for (int t = threadIdx.x; t < (H / 2) * W; t += NT) {
         const int l = t / W, k = t % W;
@@ -173,7 +173,7 @@ __device__ __forceinline__ void inv_plane(const float2 *g, float2 *cols, float2 
         }
         lines[l * LX + k] = v;
     }
-    __syncthreads();
+    lds_barrier();
     fft_lds_batch<W, true, H / 2, LX, NT>(lines, cols, twW);
 }
 

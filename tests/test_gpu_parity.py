@@ -395,8 +395,10 @@ def test_rejects_bad_input():
     from lenia_amd.engine import Engine
     eng = Engine()
     w = WL.plain32(seed=1)
-    with pytest.raises(L.LeniaHipError, match="not built"):
+    with pytest.raises(L.LeniaHipError, match="not built: sides must be divisible by 4"):
         eng.add_world((30, 30, 30), 3, w["kernels"], w["cfg"])
+    with pytest.raises(L.LeniaHipError, match="not built into this library and has no shape module"):
+        eng.add_world((60, 8, 60), 3, w["kernels"], w["cfg"])
     bad = [dict(k) for k in w["kernels"]]
     bad[0]["c0"] = 5
     with pytest.raises(L.LeniaHipError, match="channel out of range"):
