@@ -162,13 +162,17 @@ __device__ __forceinline__ const float &el(const float *base, unsigned idx) {
 
 // The A value alpha reads per channel (PREALPHA index quirk applied; 0 when the quirk's
 // index falls outside the channels), loaded by the sweep one plane ahead of its use.
-template <int CM>
+// With SRTL, for an SRT world moved in the sweep, also the cell's moved mass per channel
+// and its GA and S (the transport term's and the end-of-step transform's inputs), so that
+// no load of the cell's own values waits behind the previous cell's stores.
+template <int CM, bool SRTL>
 struct LocalIn {
     float aal[CM];
+    float arr[SRTL ? CM : 1], ga[SRTL ? CM : 1], s;
 };
 
-template <int W, int H, int D, int CM>
-__device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w, long long i, LocalIn<CM> &in) {
+template <int W, int H, int D, int CM, bool SRTL>
+__device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w, long long i, LocalIn<CM, SRTL> &in) {
     using S = Shape<W, H, D>;
     const int C = w.C;
     const float *A = A_cur(a, w);
@@ -181,6 +185,18 @@ __device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w
         bool ok = c < C;
         if (qa) { cs = r / (int)S::P; is = r - cs * (int)S::P; ok = ok && cs < C && is < S::N; }
         in.aal[c] = ok ? el(A, (unsigned)(cs * S::N + is)) : 0.0f;
+    }
+    if constexpr (SRTL) {
+        if (srt_local_world(w)) {
+            const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A : a.pool + w.Bk;
+            const float *GA = a.pool + w.GA;
+#pragma unroll
+            for (int c = 0; c < CM; c++) {
+                in.arr[c] = c < C ? el(ARR, (unsigned)(c * S::N + i)) : 0.0f;
+                in.ga[c] = c < C ? el(GA, (unsigned)(c * S::N + i)) : 0.0f;
+            }
+            in.s = (w.quirks & LHQ_SUM_RESET) ? 0.0f : el(S_cur(a, w), (unsigned)i);
+        }
     }
 }
 
@@ -195,13 +211,12 @@ __device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w
 // SRTL: compiled with that path (a launch whose shape class holds such worlds).
 template <int W, int H, int D, int CM, bool SRTL, class Grad>
 __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
-                                           const LocalIn<CM> &in, Grad grad, float (&tt)[CM]) {
+                                           const LocalIn<CM, SRTL> &in, Grad grad, float (&tt)[CM]) {
     using S = Shape<W, H, D>;
     const int C = w.C;
     const float *A = A_cur(a, w);
     const bool srt = SRTL && srt_local_world(w), store_mu = !srt || w.keep_mu;
     const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
-    const float *ARR = (w.growth_type == 0 || w.growth_type == 2) ? A : a.pool + w.Bk;
     const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
     const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
     const bool wall = w.border == 1;
@@ -271,7 +286,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
                     dv = w.si - dv;
                     mm[d] = fmin_ref(fmax_ref(dv, 0.0f), w.clip);
                 }
-                float wt = el(ARR, (unsigned)(c * S::N) + iu) * mm[0];
+                float wt = in.arr[SRTL ? c : 0] * mm[0];
                 wt = wt * mm[1];
                 wt = wt * mm[2];
                 wt = wt * w.sn;
@@ -680,8 +695,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
     const bool srt = SRTL && srt_local_world(w);
     const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
-    LocalIn<CM> in;
-    local_load<W, H, D, CM>(a, w, col + (long long)W * H * z0, in);
+    LocalIn<CM, SRTL> in;
+    local_load<W, H, D, CM, SRTL>(a, w, col + (long long)W * H * z0, in);
     for (int z = z0; z < z1; z++) {
         lds_barrier();  // the plane slot about to be refilled is no longer read
         store(z + 1);
@@ -689,8 +704,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
         lds_barrier();
         const float *p0 = ring + (z % 3) * F * T::SP, *p1 = ring + ((z + 1) % 3) * F * T::SP,
                     *p2 = ring + ((z + 2) % 3) * F * T::SP;
-        const LocalIn<CM> cur = in;
-        if (z + 1 < z1) local_load<W, H, D, CM>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
+        const LocalIn<CM, SRTL> cur = in;
+        if (z + 1 < z1) local_load<W, H, D, CM, SRTL>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
         float tt[CM];
         local_cell<W, H, D, CM, SRTL>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
             const float *const pl[3] = {p0 + f * T::SP, p1 + f * T::SP, p2 + f * T::SP};
@@ -701,11 +716,14 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
                 cdiff_planes<T::SX>(pl, cx, cy, g);
             }
         }, tt);
-        if (srt) {  // the world's transport is this cell's own: the end-of-step transform here
-            const long long i = col + (long long)W * H * z;
-            TrmIn<CM> ti;
-            trm_load<W, H, D, CM>(a, w, i, ti);
-            trm_store<W, H, D, CM>(a, w, i, tt, ti);
+        if constexpr (SRTL) {
+            if (srt) {  // the world's transport is this cell's own: the end-of-step transform here
+                TrmIn<CM> ti;
+#pragma unroll
+                for (int c = 0; c < CM; c++) ti.ga[c] = cur.ga[c];
+                ti.s = cur.s;
+                trm_store<W, H, D, CM>(a, w, col + (long long)W * H * z, tt, ti);
+            }
         }
     }
 }
