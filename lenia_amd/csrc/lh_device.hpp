@@ -638,9 +638,12 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
     const float *srcS = blur_src ? a.pool + w.SUMS : S_cur(a, w);
     const float *srcH = a.pool + (blur_src ? w.US : w.Hg);
     const int nitems = (C + 1) * T::SP;
-    int fld[LI], xy[LI], pqxy[LI];
-    const float *fsrc[LI];  // the staged field each item reads (S or H_c)
-    bool ok_in[LI], ok_out[LI];
+    // per staged item (field f, slot q): one 32-bit element offset of its (x, y) source
+    // from the world's first field (every field lies after it) and two bits, readable with
+    // z inside / outside the world -- kept small so the sweep holds few registers
+    const float *wbase = a.pool + w.A;
+    const unsigned offS = (unsigned)(srcS - wbase), offH = (unsigned)(srcH - wbase);
+    unsigned off[LI], okb = 0;
 #pragma unroll
     for (int k = 0; k < LI; k++) {
         const int item = tid + k * T::THREADS;
@@ -649,32 +652,34 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
         const int xr = halo_axis(tx0 + q % T::SX - 1, W, kind, ox), yr = halo_axis(ty0 + q / T::SX - 1, H, kind, oy);
         const int m = ox + oy;
         const bool valid = item < nitems;
-        ok_in[k] = valid && (m == 0 || (kind != 2 && !(faces && m >= 2)));
-        ok_out[k] = valid && kind != 2 && !(faces && m + 1 >= 2);
-        fld[k] = f;
-        xy[k] = xr + W * yr;
-        pqxy[k] = (xr + PADR) + S::PW * (yr + PADR);
-        fsrc[k] = f == 0 ? srcS : srcH + (long long)(f > 0 ? f - 1 : 0) * S::N;
+        const bool ok_in = valid && (m == 0 || (kind != 2 && !(faces && m >= 2)));
+        const bool ok_out = valid && kind != 2 && !(faces && m + 1 >= 2);
+        okb |= (ok_in ? 1u : 0u) << (2 * k) | (ok_out ? 2u : 0u) << (2 * k);
+        off[k] = (f == 0 ? offS : offH + (unsigned)((f > 0 ? f - 1 : 0) * S::N)) + (unsigned)(xr + W * yr);
     }
     float pf[LI];
     auto fetch = [&](int zp) {
         int oz;
         const int zr = halo_axis(zp, D, kind, oz);
-        const int zoff = W * H * zr, pqz = S::PW * S::PH * (zr + PADR);
+        const unsigned zoff = (unsigned)(W * H * zr);
         if (cdcopy) {
+            const int pqz = S::PW * S::PH * (zr + PADR);
 #pragma unroll
             for (int k = 0; k < LI; k++) {
-                const int f = fld[k];
-                const bool cut = f > 0 && (long long)(pqxy[k] + pqz) + (long long)(f - 1) * S::P >= 5 * S::P / 2;
-                const bool ok = (oz ? ok_out[k] : ok_in[k]) && !cut;
-                const float *src = f == 0 ? srcS : srcH + (long long)(f - 1) * S::N;
-                pf[k] = ok ? src[xy[k] + zoff] : 0.0f;
+                const int item = tid + k * T::THREADS;
+                const int f = item / T::SP, q = item % T::SP;
+                int ox, oy;
+                const int xr = halo_axis(tx0 + q % T::SX - 1, W, kind, ox), yr = halo_axis(ty0 + q / T::SX - 1, H, kind, oy);
+                const int pqxy = (xr + PADR) + S::PW * (yr + PADR);
+                const bool cut = f > 0 && (long long)(pqxy + pqz) + (long long)(f - 1) * S::P >= 5 * S::P / 2;
+                const bool ok = ((okb >> (2 * k + (oz ? 1 : 0))) & 1u) && !cut;
+                pf[k] = ok ? el(wbase, off[k] + zoff) : 0.0f;
             }
         } else {
 #pragma unroll
             for (int k = 0; k < LI; k++) {
-                const bool ok = oz ? ok_out[k] : ok_in[k];
-                pf[k] = ok ? el(fsrc[k], (unsigned)(xy[k] + zoff)) : 0.0f;
+                const bool ok = (okb >> (2 * k + (oz ? 1 : 0))) & 1u;
+                pf[k] = ok ? el(wbase, off[k] + zoff) : 0.0f;
             }
         }
     };

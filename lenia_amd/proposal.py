@@ -142,19 +142,60 @@ def targets(r: dict) -> np.ndarray:
 FLOPS_PER_SET = 2 * (81 * 256 + 256 * 256 * 2 + 256 * 3)  # forward, algorithmic
 
 
+def save_labels(path: str, r: dict) -> None:
+    """Labels of a range of rollouts (the features are rebuilt from the set indices)."""
+    np.savez(path, survival=r["survival"], concentration=r["concentration"], drift=r["drift"])
+
+
+def load_labels(paths, seed: int, rollouts: int) -> dict:
+    """Rollout data for sets 0..rollouts-1 from label files named labels_<lo>_<hi>.npz
+    (written by save_labels for sets lo..hi-1); features rebuilt from the sets."""
+    import os
+    import re
+    parts = []
+    for p in paths:
+        m = re.search(r"labels_(\d+)_(\d+)\.npz$", os.path.basename(p))
+        if not m:
+            raise ValueError(f"{p}: expected a file named labels_<lo>_<hi>.npz")
+        parts.append((int(m.group(1)), int(m.group(2)), p))
+    parts.sort()
+    out = {k: np.empty(rollouts, np.float32) for k in ("survival", "concentration", "drift")}
+    nxt = 0
+    for lo, hi, p in parts:
+        if lo != nxt:
+            raise ValueError(f"label files leave sets {nxt}..{lo - 1} uncovered")
+        z = np.load(p)
+        n = min(hi, rollouts) - lo
+        for k in out:
+            out[k][lo:lo + n] = z[k][:n]
+        nxt = hi
+        if nxt >= rollouts:
+            break
+    if nxt < rollouts:
+        raise ValueError(f"label files cover {nxt} of {rollouts} sets")
+    out["features"] = features(sample(seed, np.arange(rollouts)))
+    return out
+
+
 def pipeline(rollouts: int, steps: int, score: int, top: int, epochs: int, batch: int = 256, lr: float = 1e-3,
-             seed: int = 100, log=print) -> dict:
+             seed: int = 100, log=print, labels=None) -> dict:
     """Rollouts -> training -> scoring `score` fresh sets in one call -> re-simulating the
-    `top` best and `top` uniform sets; returns the measurements."""
+    `top` best and `top` uniform sets; returns the measurements.  labels: files written
+    by save_labels for sets 0..rollouts-1 of the same seed and steps, used instead of
+    running those rollouts again."""
     import time
 
     from . import mlp as M
     out = dict(rollouts=rollouts, steps=steps, scored=score, top=top, epochs=epochs, batch=batch, seed=seed)
     t0 = time.perf_counter()
-    data = rollout(seed, np.arange(rollouts), steps,
-                   progress=lambda n: log(f"rollouts {n}/{rollouts} ({time.perf_counter() - t0:.0f} s)"))
-    t_roll = time.perf_counter() - t0
-    out["rollout_world_steps_per_s"] = rollouts * steps / t_roll
+    if labels:
+        data = load_labels(labels, seed, rollouts)
+        out["labels_from"] = len(labels)
+    else:
+        data = rollout(seed, np.arange(rollouts), steps,
+                       progress=lambda n: log(f"rollouts {n}/{rollouts} ({time.perf_counter() - t0:.0f} s)"))
+        t_roll = time.perf_counter() - t0
+        out["rollout_world_steps_per_s"] = rollouts * steps / t_roll
     out["train_base_rate"] = float(data["survival"].mean())
     x, y = data["features"], targets(data)
     nval = rollouts // 10
@@ -201,8 +242,27 @@ def main():
     ap.add_argument("--top", type=int, default=4096)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--seed", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--labels", nargs="*", default=None,
+                    help="label files of earlier rollouts (labels_<lo>_<hi>.npz, sets lo..hi-1) to train on "
+                         "instead of running the rollouts")
+    ap.add_argument("--rollout-range", nargs=2, type=int, default=None, metavar=("LO", "HI"),
+                    help="only run rollouts of sets LO..HI-1 and write labels_<LO>_<HI>.npz (to --out)")
+    ap.add_argument("--out", default=".")
     a = ap.parse_args()
-    r = pipeline(a.rollouts, a.steps, a.score, a.top, a.epochs, seed=a.seed, log=lambda s: print(s, flush=True))
+    if a.rollout_range:
+        import os
+        import time
+        lo, hi = a.rollout_range
+        t0 = time.perf_counter()
+        r = rollout(a.seed, np.arange(lo, hi), a.steps,
+                    progress=lambda n: print(f"rollouts {lo + n}/{hi} ({time.perf_counter() - t0:.0f} s)", flush=True))
+        save_labels(os.path.join(a.out, f"labels_{lo}_{hi}.npz"), r)
+        print(json.dumps(dict(sets=[lo, hi], steps=a.steps, seconds=time.perf_counter() - t0,
+                              survival=float(r["survival"].mean()))), flush=True)
+        return
+    r = pipeline(a.rollouts, a.steps, a.score, a.top, a.epochs, batch=a.batch, seed=a.seed,
+                 log=lambda s: print(s, flush=True), labels=a.labels)
     print(json.dumps(r), flush=True)
 
 
