@@ -792,11 +792,12 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
     const int cx = tid % T::TX, cy = tid / T::TX;  // this thread's destination column
     // z-independent part of every staged item: offsets of its (x, y) source and whether
     // it is readable with z inside / outside the world (halo_src split per axis)
-    unsigned off[T::ITEMS];
-    bool ok_in[T::ITEMS], ok_out[T::ITEMS];
-    // destination centres toward which an item's weights are formed, slot + 1 - kk for
-    // kk = 0, 1, 2 (exact: small integers + 0.5)
-    float px[T::ITEMS][3], py[T::ITEMS][3], shx[T::ITEMS], shy[T::ITEMS];
+    // (few registers per item: the destination centres slot + 1 - kk are formed from the
+    // slot's own centre, exact since both are small integers + 0.5, and the two flags
+    // share one word)
+    static_assert(2 * T::ITEMS <= 32, "readability flags exceed one word");
+    unsigned off[T::ITEMS], okb = 0;
+    float xc[T::ITEMS], yc[T::ITEMS], shx[T::ITEMS], shy[T::ITEMS];
 #pragma unroll
     for (int k = 0; k < T::ITEMS; k++) {
         const int item = tid + k * T::THREADS;
@@ -806,14 +807,12 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
         const int xr = halo_axis(x, W, tkind, ox), yr = halo_axis(y, H, tkind, oy);
         const int m = ox + oy;
         const bool valid = item < nitems;
-        ok_in[k] = valid && (m == 0 || (tkind != 2 && !(faces && m >= 2)));
-        ok_out[k] = valid && tkind != 2 && !(faces && m + 1 >= 2);
+        const bool ok_in = valid && (m == 0 || (tkind != 2 && !(faces && m >= 2)));
+        const bool ok_out = valid && tkind != 2 && !(faces && m + 1 >= 2);
+        okb |= (ok_in ? 1u : 0u) << (2 * k) | (ok_out ? 2u : 0u) << (2 * k);
         off[k] = valid ? (unsigned)(c * 3 * S::N + xr + W * yr) : 0u;  // MU of channel c; mass at c*N
-#pragma unroll
-        for (int kk = 0; kk < 3; kk++) {
-            px[k][kk] = (float)(x + 1 - kk) + 0.5f;
-            py[k][kk] = (float)(y + 1 - kk) + 0.5f;
-        }
+        xc[k] = (float)x + 0.5f;
+        yc[k] = (float)y + 0.5f;
         shx[k] = !torus ? 0.0f : (x < 0 ? (float)W : (x >= W ? -(float)W : 0.0f));
         shy[k] = !torus ? 0.0f : (y < 0 ? (float)H : (y >= H ? -(float)H : 0.0f));
     }
@@ -823,7 +822,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
         const unsigned zoff = (unsigned)(W * H * zr);
 #pragma unroll
         for (int k = 0; k < T::ITEMS; k++) {
-            const bool ok = oz ? ok_out[k] : ok_in[k];
+            const bool ok = (okb >> (2 * k + (oz ? 1 : 0))) & 1u;
             const unsigned o = off[k] + zoff;
             const int c = (tid + k * T::THREADS) / T::SP;
 #pragma unroll
@@ -858,7 +857,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) voi
 #pragma unroll
                 for (int kk = 0; kk < 3; kk++) {
                     // destination coordinate slot + 1 - kk, centre (float)(dest) + 0.5
-                    const float pos = d == 0 ? px[k][kk] : (d == 1 ? py[k][kk] : pz[kk]);
+                    const float pos = d == 0 ? xc[k] + (float)(1 - kk) : (d == 1 ? yc[k] + (float)(1 - kk) : pz[kk]);
                     float dv = pos - mu;
                     dv = dv + sh[d];
                     dv = si - __builtin_fabsf(dv);
