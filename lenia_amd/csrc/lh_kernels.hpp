@@ -138,22 +138,39 @@ __global__ __launch_bounds__(128) void k_spec_z(PhaseArgs a, int mode) {
         const float2 *src = Fsp + (long long)c * S::Nh + ky * Wh + kx;
 #pragma unroll
         for (int z = 0; z < D; z++) t[z] = src[(long long)z * H * Wh];
-        fft_line<D, false>(t, a.twD);
         const int nk = mode == 0 ? w.K : 1;
-        for (int k = 0; k < nk; k++) {
-            if (mode == 0 && w.kern[k].c0 != c) continue;
+        // the octant values of the kernels reading this channel, each kernel's loaded
+        // before the previous kernel's stores (a load issued after a store waits for it:
+        // loads and stores share one completion counter)
+        auto next_k = [&](int k) {
+            for (k = k + 1; k < nk; k++)
+                if (mode != 0 || w.kern[k].c0 == c) return k;
+            return nk;
+        };
+        float kv[S::Dh];
+        auto load_oct = [&](int k) {
+            if (k >= nk) return;
             const float *oct = mode == 0 ? a.pool + w.KH + (long long)k * S::Noct : a.pool + w.BH;
+#pragma unroll
+            for (int zo = 0; zo < S::Dh; zo++) kv[zo] = oct[(zo * Hh + kyo) * Wh + kx];
+        };
+        int k = next_k(-1);
+        load_oct(k);
+        fft_line<D, false>(t, a.twD);
+        while (k < nk) {
+            const int kn = next_k(k);
             float2 y[D];
 #pragma unroll
             for (int z = 0; z < D; z++) {
                 const int zo = z <= D / 2 ? z : D - z;
-                const float kv = oct[(zo * Hh + kyo) * Wh + kx];
-                y[z] = make_float2(t[z].x * kv, t[z].y * kv);
+                y[z] = make_float2(t[z].x * kv[zo], t[z].y * kv[zo]);
             }
+            load_oct(kn);
             fft_line<D, true>(y, a.twD);
             float2 *dst = Gsp + (long long)(mode == 0 ? k : c) * S::Nh + ky * Wh + kx;
 #pragma unroll
             for (int z = 0; z < D; z++) dst[(long long)z * H * Wh] = y[z];
+            k = kn;
         }
     }
 }

@@ -121,17 +121,40 @@ __global__ __launch_bounds__(256) void k_spec_z_big(PhaseArgs a, int mode) {
             continue;
         }
         const int nk = mode == 0 ? w.K : 1;
-        for (int k = 0; k < nk; k++) {
-            if (mode == 0 && w.kern[k].c0 != c) continue;
+        // each kernel's octant values (this thread's elements) are loaded before the
+        // previous kernel's stores: a load issued after a store would wait for it
+        constexpr int EPT = (CB * D + NT - 1) / NT;
+        auto next_k = [&](int k) {
+            for (k = k + 1; k < nk; k++)
+                if (mode != 0 || w.kern[k].c0 == c) return k;
+            return nk;
+        };
+        float kv[EPT];
+        auto load_oct = [&](int k) {
+            if (k >= nk) return;
             const float *oct = mode == 0 ? a.pool + w.KH + (long long)k * S::Noct : a.pool + w.BH;
-            for (int t = threadIdx.x; t < CB * D; t += NT) {
+#pragma unroll
+            for (int e = 0; e < EPT; e++) {
+                const int t = threadIdx.x + e * NT;
                 const int z = t / CB, cb = t % CB, col = col0 + cb;
                 const int kx = col % Wh, ky = col / Wh;
                 const int kyo = ky <= H / 2 ? ky : H - ky, zo = z <= D / 2 ? z : D - z;
-                const float kv = col < COLS ? oct[(zo * Hh + kyo) * Wh + kx] : 0.0f;
-                const float2 tv = sT[cb * LZ + z];
-                sY[cb * LZ + z] = make_float2(tv.x * kv, tv.y * kv);
+                kv[e] = (t < CB * D && col < COLS) ? oct[(zo * Hh + kyo) * Wh + kx] : 0.0f;
             }
+        };
+        int k = next_k(-1);
+        load_oct(k);
+        while (k < nk) {
+            const int kn = next_k(k);
+#pragma unroll
+            for (int e = 0; e < EPT; e++) {
+                const int t = threadIdx.x + e * NT;
+                if (t >= CB * D) break;
+                const int z = t / CB, cb = t % CB;
+                const float2 tv = sT[cb * LZ + z];
+                sY[cb * LZ + z] = make_float2(tv.x * kv[e], tv.y * kv[e]);
+            }
+            load_oct(kn);
             lds_barrier();
             fft_lds_batch<D, true, CB, LZ, NT>(sY, sS, a.twD);
             float2 *dst = Gsp + (long long)(mode == 0 ? k : c) * S::Nh;
@@ -140,6 +163,7 @@ __global__ __launch_bounds__(256) void k_spec_z_big(PhaseArgs a, int mode) {
                 if (col < COLS) dst[(long long)z * COLS + col] = sY[cb * LZ + z];
             }
             lds_barrier();
+            k = kn;
         }
     }
 }
