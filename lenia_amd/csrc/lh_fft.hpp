@@ -5,7 +5,7 @@
 // register indexing inside a transform.  The schedule is Stockham autosort with the
 // radix plan 4,...,4,[2],odd primes ascending, twiddles W_n^e = (cos, -/+sin)(2 pi e/n)
 // read from a host-built table (double precision rounded once to fp32), odd radices
-// as a direct DFT accumulated in index order.  Built with -ffp-contract=off, so each
+// as paired cosine and sine sums in index order (fft_butterfly).  Built with -ffp-contract=off, so each
 // butterfly rounds exactly as written; the CPU checker restates the same schedule.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -73,17 +73,40 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[R], const float2 *__re
         v[1] = INV ? q : p;
         v[3] = INV ? p : q;
     } else {
-        float2 y[R];
+        // odd R: inputs paired as s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j} (j = 1..M,
+        // M = (R-1)/2); y_0 = v_0 + s_1 + ... + s_M; for k = 1..M the cosine part
+        // A_k = v_0 + sum_j s_j cos(2 pi jk/R) and the sine part C_k = sum_j d_j sin(2 pi jk/R),
+        // both summed in j order, give y_k = A_k -/+ i C_k, y_{R-k} = A_k +/- i C_k (upper
+        // sign forward): about a quarter of a direct DFT's products
+        constexpr int M = (R - 1) / 2;
+        float2 y[R], sv[M + 1], dv[M + 1];
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            float2 acc = v[0];
+        for (int j = 1; j <= M; j++) {
+            sv[j] = c_add(v[j], v[R - j]);
+            dv[j] = c_sub(v[j], v[R - j]);
+        }
+        y[0] = v[0];
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                const int e = (j * k) % R;
-                if (e == 0) acc = c_add(acc, v[j]);
-                else acc = c_add(acc, c_mul(v[j], twiddle<INV>(tw, e * (N / R))));
+        for (int j = 1; j <= M; j++) y[0] = c_add(y[0], sv[j]);
+#pragma unroll
+        for (int k = 1; k <= M; k++) {
+            float ar = v[0].x, ai = v[0].y, cr = 0.0f, ci = 0.0f;
+#pragma unroll
+            for (int j = 1; j <= M; j++) {
+                const float2 w = twiddle<true>(tw, ((j * k) % R) * (N / R));  // (cos, sin)
+                ar = ar + sv[j].x * w.x;
+                ai = ai + sv[j].y * w.x;
+                if (j == 1) {
+                    cr = dv[j].x * w.y;
+                    ci = dv[j].y * w.y;
+                } else {
+                    cr = cr + dv[j].x * w.y;
+                    ci = ci + dv[j].y * w.y;
+                }
             }
-            y[k] = acc;
+            const float2 p = make_float2(ar + ci, ai - cr), q = make_float2(ar - ci, ai + cr);  // A -/+ iC
+            y[k] = INV ? q : p;
+            y[R - k] = INV ? p : q;
         }
 #pragma unroll
         for (int k = 0; k < R; k++) v[k] = y[k];

@@ -48,7 +48,7 @@ static inline cpx cmul(cpx a, cpx w) {
 /* ------------------------------------------------------------------------------ */
 /* 1-D complex FFT: Stockham autosort, radix plan 4..4,[2],odd primes ascending.    */
 /* Twiddles W_n^m = (cos(2 pi m/n), -/+ sin(2 pi m/n)) computed in double, rounded  */
-/* once to REAL.  Odd radices use a direct DFT accumulated in index order.          */
+/* once to REAL.  Odd radices as paired cosine and sine sums (butterfly below).     */
 /* ------------------------------------------------------------------------------ */
 #define MAXN 256
 #define MAXR 32
@@ -90,15 +90,30 @@ static void butterfly(const fplan *f, cpx *v, int R, int inv) {
         q.r = b.r - d.i; q.i = b.i + d.r;
         if (inv) { v[1] = q; v[3] = p; } else { v[1] = p; v[3] = q; }
     } else {
-        cpx y[MAXR];
-        int step = f->n / R;
-        for (int k = 0; k < R; k++) {
-            cpx acc = v[0];
-            for (int j = 1; j < R; j++) {
-                int e = (j * k) % R;
-                acc = e == 0 ? cadd(acc, v[j]) : cadd(acc, cmul(v[j], tw(f, e * step, inv)));
+        /* odd R: inputs paired as s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j} (j = 1..M,
+         * M = (R-1)/2); y_0 = v_0 + s_1 + ... + s_M; for k = 1..M the cosine part
+         * A_k = v_0 + sum_j s_j cos(2 pi jk/R) and the sine part C_k = sum_j d_j sin(2 pi jk/R),
+         * both summed in j order, give y_k = A_k -/+ i C_k and y_{R-k} = A_k +/- i C_k
+         * (upper sign forward) */
+        cpx y[MAXR], sv[MAXR / 2 + 1], dv[MAXR / 2 + 1];
+        const int step = f->n / R, M = (R - 1) / 2;
+        for (int j = 1; j <= M; j++) { sv[j] = cadd(v[j], v[R - j]); dv[j] = csub(v[j], v[R - j]); }
+        y[0] = v[0];
+        for (int j = 1; j <= M; j++) y[0] = cadd(y[0], sv[j]);
+        for (int k = 1; k <= M; k++) {
+            real ar = v[0].r, ai = v[0].i, cr = 0, ci = 0;
+            for (int j = 1; j <= M; j++) {
+                const int e = ((j * k) % R) * step;
+                const real c = f->c[e], sn = f->s[e];
+                ar = ar + sv[j].r * c;
+                ai = ai + sv[j].i * c;
+                if (j == 1) { cr = dv[j].r * sn; ci = dv[j].i * sn; }
+                else { cr = cr + dv[j].r * sn; ci = ci + dv[j].i * sn; }
             }
-            y[k] = acc;
+            cpx p, q; /* p = A - iC, q = A + iC */
+            p.r = ar + ci; p.i = ai - cr;
+            q.r = ar - ci; q.i = ai + cr;
+            if (inv) { y[k] = q; y[R - k] = p; } else { y[k] = p; y[R - k] = q; }
         }
         for (int k = 0; k < R; k++) v[k] = y[k];
     }

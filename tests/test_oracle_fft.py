@@ -7,7 +7,7 @@ restatement's transforms are pinned by the mathematics instead.
 import numpy as np
 import pytest
 
-SIZES = [2, 3, 4, 5, 6, 8, 9, 11, 12, 16, 24, 32, 36, 40, 44, 48, 64, 96, 128]
+SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 16, 17, 24, 28, 31, 32, 36, 40, 44, 48, 52, 64, 96, 124, 128]
 SHAPES = [(4, 2, 4), (8, 4, 8), (8, 8, 8), (12, 12, 12), (16, 8, 16), (24, 24, 24), (32, 32, 32),
           (36, 36, 36), (44, 24, 44), (48, 48, 48)]
 
@@ -22,7 +22,7 @@ def test_fft1_matches_numpy(oracle, n, prec, tol):
     assert np.abs(oracle.fft1(x, True, prec) - np.fft.ifft(x) * n).max() <= tol * scale
 
 
-@pytest.mark.parametrize("n", [2, 3, 4, 6, 8, 12])
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 28])
 def test_fft1_brute_force_dft(oracle, n):
     rng = np.random.default_rng(100 + n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
