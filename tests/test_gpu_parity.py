@@ -314,6 +314,57 @@ def test_mixed_batch_4096_hundred_steps_one_launch():
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_step_graph_cache_evicts_and_stays_exact():
+    """More distinct step counts than graphs kept (least recently used evicted), odd
+    and even counts (the A/S buffer parity flips), against single eager steps."""
+    from lenia_amd import _lib as L
+    from lenia_amd.engine import Engine
+    ws = WL.mixed(n_worlds=4, base_seed=5200, asymptotic=ASYM)
+    counts = [3, 1, 4, 2, 5, 7, 6, 8, 3, 1]
+    outs = []
+    for graph in (True, False):
+        eng = Engine(graph=graph)
+        for w in ws:
+            eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        for i, w in enumerate(ws):
+            eng.set(i, "A", w["A"])
+        snaps = []
+        for n in counts:
+            if graph:
+                eng.step(n)
+            else:
+                for _ in range(n):
+                    eng.step(1)
+            eng.synchronize()
+            snaps.append([eng.get(i, "A") for i in range(len(ws))] + [eng.get(i, "S") for i in range(len(ws))])
+        if graph:
+            with pytest.raises(L.LeniaHipError, match="nsteps"):
+                eng.prepare_steps(0)
+            with pytest.raises(L.LeniaHipError, match="nsteps"):
+                eng.prepare_steps(257)
+        eng.close()
+        outs.append(snaps)
+    for a_snap, b_snap in zip(*outs):
+        for a, b in zip(a_snap, b_snap):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_srt_mu_needs_keep_mu():
+    """SRT worlds move inside the gradient sweep and do not store mu unless asked."""
+    from lenia_amd import _lib as L
+    from lenia_amd.engine import Engine
+    w = WL.plain32(seed=3, mode="srt")
+    eng = Engine()
+    wid = eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+    eng.commit()
+    eng.set(wid, "A", w["A"])
+    eng.step(1)
+    with pytest.raises(L.LeniaHipError, match="LH_OPT_KEEP_MU"):
+        eng.get(wid, "MU")
+    eng.close()
+
+
 def test_large_batch_one_thread_per_column_spectral_path():
     """128 worlds of 32^3 (69,632 spectral columns) take spec_z's one-thread-per-column
     schedule; smaller batches take the per-(column, kernel) one.  Spot-checked worlds of
