@@ -226,7 +226,8 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2c(PhaseArgs a, int nf)
     load_matrix<N, SPLIT>(a.dft, IO == 0 ? M_FI : M_GI, rb0, lane, ih, il);
     fetch(id);
     for (;;) {
-        __syncthreads();  // the previous tile's products have read the staged tile
+        lds_barrier();  // the previous tile's products have read the staged tile (LDS only:
+                        // the next tile's loads stay in flight, tiles are disjoint lines)
 #pragma unroll
         for (int it = 0; it < G::ITER; it++) {
             const int l = tid & 31, q = (tid >> 5) + it * (G::THREADS / 32);
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2c(PhaseArgs a, int nf)
         const int cur = id;
         id = next_valid(id + gridDim.x);
         if (id < total) fetch(id);  // in flight during this tile's products
-        __syncthreads();
+        lds_barrier();
         f32x4 cr[G::RB][G::LB], ci[G::RB][G::LB], xr[G::RB][G::LB], xi[G::RB][G::LB];
 #pragma unroll
         for (int rb = 0; rb < G::RB; rb++)
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_r2c_x(PhaseArgs a, int n
     u16x8 mh[G::RB][G::KS], ml[G::RB][G::KS];
     const int rb0 = wave * G::RB;
     load_matrix<N, SPLIT>(a.dft, M_XF, rb0, lane, mh, ml);
-    __syncthreads();
+    lds_barrier();
     f32x4 acc[G::RB][G::LB];
     gemm_real<N, T, SPLIT>(sb[0], sb[1], lane, mh, ml, 1.0f / N, acc);
 #pragma unroll
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2r_growth(PhaseArgs a, 
     constexpr int QC = N / 8;
     for (int k = 0; k < K; k++) {
         const float *src = a.pool + w.Gsp + k * G::SPEC;
-        __syncthreads();  // previous kernel's tile fully consumed (and the LUT staged)
+        lds_barrier();  // previous kernel's tile fully consumed (and the LUT staged)
 #pragma unroll
         for (int it = 0; it < G::ITER; it++) {
             const int q = tid % QC, l = tid / QC + it * (G::THREADS / QC);
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2r_growth(PhaseArgs a, 
             sts8(&sb[0][l * G::LDK + q * 8], h);
             if (SPLIT) sts8(&sb[1][l * G::LDK + q * 8], lo);
         }
-        __syncthreads();
+        lds_barrier();
         f32x4 u[G::RB][G::LB];
         gemm_real<N, T, SPLIT>(sb[0], sb[1], lane, mh, ml, 1.0f, u);
         const DevKernel kp = w.kern[k];
@@ -449,10 +450,22 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2r_growth(PhaseArgs a, 
                 }
     }
     float *Hg = a.pool + w.Hg, *B = a.pool + w.Bk;
+    const bool with_b = w.growth_type == 1;
 #pragma unroll
     for (int c = 0; c < LHD_MAX_C; c++) {
         if (c >= C) break;
         const bool soft = (w.soft_mask >> c) & 1;
+        // the channel's A values first, all in flight before its stores (a load issued
+        // after a store waits for it: loads and stores share one completion counter)
+        float av[G::RB][G::LB][4];
+        if (with_b) {
+#pragma unroll
+            for (int rb = 0; rb < G::RB; rb++)
+#pragma unroll
+                for (int lb = 0; lb < G::LB; lb++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) av[rb][lb][i] = A[c * G::N3 + cell(rb, lb, i)];
+        }
 #pragma unroll
         for (int rb = 0; rb < G::RB; rb++)
 #pragma unroll
@@ -462,8 +475,9 @@ __global__ __launch_bounds__(GC<N>::THREADS) void k_dft_c2r_growth(PhaseArgs a, 
                     const long long gi = c * G::N3 + cell(rb, lb, i);
                     const float hv = hacc[c][rb][lb][i];
                     Hg[gi] = hv;
-                    if (w.growth_type == 1)
-                        B[gi] = soft ? soft_growth(A[gi], hv, w.dt) : clamp_ref(A[gi] + hv * w.dt, 0.0f, 1.0f);
+                    if (with_b)
+                        B[gi] = soft ? soft_growth(av[rb][lb][i], hv, w.dt)
+                                     : clamp_ref(av[rb][lb][i] + hv * w.dt, 0.0f, 1.0f);
                 }
     }
 }
