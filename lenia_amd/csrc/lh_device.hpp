@@ -551,19 +551,35 @@ inline int sweep_zseg(int nworlds, int blocks_per_world, int depth) {
 
 // Geometry of the z-sweep kernels (k_local_sweep, k_transport_sweep): a workgroup owns
 // a TX x TY column of cells through the whole depth and stages one z-plane at a time.
-template <int W, int H, int D, int CM, int TYMAX = 8>
+// Rows per sweep workgroup: the divisor of H with the least overhead, (TY + 2) / TY rows
+// staged per row moved (the y halo) times the lanes of whole waves per lane used, for at
+// most MAXT threads and the transport's plane -- ten floats per slot and channel -- within
+// 64 KB of LDS.
+constexpr int sweep_ty(int TX, int H, int CM, int MAXT) {
+    int best = 1;
+    long long bnum = 0, bden = 1;  // best cost as a fraction
+    for (int ty = 1; ty <= H; ty++) {
+        if (H % ty) continue;
+        const int thr = TX * ty;
+        if ((ty > 1 && thr > MAXT) || CM * 10 * (TX + 2) * (ty + 2) * 4 > 65536) break;
+        const long long num = (long long)(ty + 2) * ((thr + 63) / 64 * 64), den = (long long)ty * thr;
+        if (ty == 1 || num * bden < bnum * den) { best = ty; bnum = num; bden = den; }
+    }
+    return best;
+}
+
+template <int W, int H, int D, int CM, int MAXT>
 struct SweepCfg {
     static constexpr int TX = stencil_tx(W);
     static constexpr int XT = W / TX;
-    // 8 rows when the plane's weights for 4 channels fit 64 KB of LDS, else 4
-    static constexpr int TY = (TYMAX >= 8 && TX <= 38 && H % 8 == 0) ? 8 : (H % 4 == 0 ? 4 : (H % 2 == 0 ? 2 : 1));
+    static constexpr int TY = sweep_ty(TX, H, CM, MAXT);
     static constexpr int YT = H / TY;
     static constexpr int THREADS = TX * TY;
     static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
     static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
     static constexpr int BLOCKS = XT * YT;
     static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // arr, wx0..2, wy0..2, wz0..2
-    static_assert(LHD_MAX_C * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
+    static_assert(CM * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
 };
 
 // One coordinate through the transport halo (halo_src, axis by axis): resolved index
@@ -615,10 +631,12 @@ __device__ __forceinline__ void cdiff_planes(const float *const (&pl)[3], int tx
 // HBM once per column instead of once per tile it borders.
 // zseg: planes per workgroup (the depth is split into segments when few worlds would
 // leave the GPU idle).
+// Thread cap of a gradient-sweep workgroup: four waves, as the transport sweep's.
+constexpr int LOCAL_MAXT = 256;
 template <int W, int H, int D, int CM, bool SRTL>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_sweep(PhaseArgs a, int zseg) {
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS)) void k_local_sweep(PhaseArgs a, int zseg) {
     using S = Shape<W, H, D>;
-    using T = SweepCfg<W, H, D, CM>;
+    using T = SweepCfg<W, H, D, CM, LOCAL_MAXT>;
     constexpr int F = CM + 1;  // staged fields: S, H_0 .. H_{C-1}
     constexpr int LI = (F * T::SP + T::THREADS - 1) / T::THREADS;
     __shared__ float ring[3 * F * T::SP];
@@ -737,7 +755,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM>::THREADS)) void k_local_swee
 // moved inside the sweep, maxC << 8, planes per workgroup << 16 (0: sweep_zseg).
 template <int W, int H, int D, int CM, bool SRTL>
 void launch_local_sweep_cm(const PhaseArgs &a, int arg, hipStream_t st) {
-    using SC = SweepCfg<W, H, D, CM>;
+    using SC = SweepCfg<W, H, D, CM, LOCAL_MAXT>;
     const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
     hipLaunchKernelGGL((k_local_sweep<W, H, D, CM, SRTL>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
                        st, a, zs);
@@ -761,14 +779,16 @@ void launch_local_sweep(const PhaseArgs &a, int arg, hipStream_t st) {
 // complete after step z', when its TRM runs.  The product arr*wx*wy of a source is the
 // same for the three destinations and is formed once.
 // ------------------------------------------------------------------------------------
-// CM: channels moved, at most (3 or LHD_MAX_C).  Column height (rows): 8 measured
-// faster than 4 on the 32^3 batch.
-constexpr int TRANSPORT_TY = 8;
+// CM: channels moved, at most (3 or LHD_MAX_C).  Thread cap of a workgroup (sweep_ty picks
+// its rows): four waves.  On batches of 1,024 worlds, 36^3 moved from 4 rows to 6 under
+// this cap (835 -> 766 us, the gradient sweep 1491 -> 1300 us); caps of 320 and 448 (six
+// to eight rows of 44 cells on 44 x 24 x 44, twelve rows on 36^3) ran slower.
+constexpr int TRANSPORT_MAXT = 256;
 template <int W, int H, int D, int CM>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_TY>::THREADS)) void k_transport_sweep(PhaseArgs a,
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) void k_transport_sweep(PhaseArgs a,
                                                                                                    int zseg) {
     using S = Shape<W, H, D>;
-    using T = SweepCfg<W, H, D, CM, TRANSPORT_TY>;
+    using T = SweepCfg<W, H, D, CM, TRANSPORT_MAXT>;
     extern __shared__ float sw_dyn[];
     // destination planes [z0, z1) of this workgroup; source planes z0-1 .. z1
     const int nseg = (D + zseg - 1) / zseg;
