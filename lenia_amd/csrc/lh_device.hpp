@@ -633,8 +633,19 @@ __device__ __forceinline__ void cdiff_planes(const float *const (&pl)[3], int tx
 // leave the GPU idle).
 // Thread cap of a gradient-sweep workgroup: four waves, as the transport sweep's.
 constexpr int LOCAL_MAXT = 256;
+// Waves per SIMD the gradient sweep is compiled for (its registers: at most 80 VGPRs for
+// six, 96 for five): for three channels in workgroups of four waves, six waves without the
+// in-sweep SRT transform (32^3: 85 -> 80 VGPRs, 2.17 -> 2.03 ms on 4,096 worlds) and five
+// with it (a few registers spilled; 1,024 worlds: 32^3 797 -> 681 us, 36^3 1300 -> 1181
+// us); workgroups of three waves (24^3, 44 x 24 x 44) ran slower so (372 -> 406 us,
+// 1273 -> 1454 us) and keep what the compiler picks.
 template <int W, int H, int D, int CM, bool SRTL>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS)) void k_local_sweep(PhaseArgs a, int zseg) {
+constexpr int local_sweep_waves() {
+    return CM == 3 && (SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS + 63) / 64 == 4 ? (SRTL ? 5 : 6) : 1;
+}
+template <int W, int H, int D, int CM, bool SRTL>
+__global__ __launch_bounds__((SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS))
+__attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL>()))) void k_local_sweep(PhaseArgs a, int zseg) {
     using S = Shape<W, H, D>;
     using T = SweepCfg<W, H, D, CM, LOCAL_MAXT>;
     constexpr int F = CM + 1;  // staged fields: S, H_0 .. H_{C-1}
@@ -714,7 +725,7 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS)) void 
     fetch(z0);
     store(z0);
     fetch(z0 + 1);
-    const int cx = tid % T::TX, cy = tid / T::TX;
+    const int cx = tid % T::TX, cy = tid / T::TX, toff = cx + T::SX * cy;
     const bool perm = (w.quirks & LHQ_SOBEL_PERMUTE) != 0;
     const bool srt = SRTL && srt_local_world(w);
     const long long col = (tx0 + cx) + (long long)W * (ty0 + cy);
@@ -725,18 +736,25 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS)) void 
         store(z + 1);
         if (z + 2 <= z1) fetch(z + 2);
         lds_barrier();
-        const float *p0 = ring + (z % 3) * F * T::SP, *p1 = ring + ((z + 1) % 3) * F * T::SP,
-                    *p2 = ring + ((z + 2) % 3) * F * T::SP;
+        // element offsets of the three planes (z - 1, z, z + 1) in the ring
+        const int r0 = (z % 3) * F * T::SP, r1 = ((z + 1) % 3) * F * T::SP, r2 = ((z + 2) % 3) * F * T::SP;
         const LocalIn<CM, SRTL> cur = in;
         if (z + 1 < z1) local_load<W, H, D, CM, SRTL>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
         float tt[CM];
         local_cell<W, H, D, CM, SRTL>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
-            const float *const pl[3] = {p0 + f * T::SP, p1 + f * T::SP, p2 + f * T::SP};
+            // the corner of this thread's 3 x 3 neighbourhood in each plane of field f, each
+            // offset opaque to the compiler (an empty statement that claims to change it), so
+            // that the nine reads of a plane pair up off one address register; without it
+            // reads of different planes and fields are paired, each pair needing an address
+            // register of its own (one VALU add per pair)
+            int o[3] = {r0 + f * T::SP + toff, r1 + f * T::SP + toff, r2 + f * T::SP + toff};
+            asm volatile("" : "+v"(o[0]), "+v"(o[1]), "+v"(o[2]));
+            const float *const pl[3] = {ring + o[0], ring + o[1], ring + o[2]};
             if (w.gradient == 1) {
-                if (perm) sobel_planes<T::SX, true>(pl, cx, cy, g);
-                else sobel_planes<T::SX, false>(pl, cx, cy, g);
+                if (perm) sobel_planes<T::SX, true>(pl, 0, 0, g);
+                else sobel_planes<T::SX, false>(pl, 0, 0, g);
             } else {
-                cdiff_planes<T::SX>(pl, cx, cy, g);
+                cdiff_planes<T::SX>(pl, 0, 0, g);
             }
         }, tt);
         if constexpr (SRTL) {
