@@ -578,7 +578,7 @@ struct SweepCfg {
     static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
     static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
     static constexpr int BLOCKS = XT * YT;
-    static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // arr, wx0..2, wy0..2, wz0..2
+    static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // transport: nine weights and the mass per slot
     static_assert(CM * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
 };
 
@@ -876,6 +876,20 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) v
         accN[c] = 0.0f;
     }
     TrmIn<CM> pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
+    // LDS per channel (10 * SP floats): the z weights of a source toward destinations
+    // z'+1 and z' as one float2, its z weight toward z'-1 and its mass as another, then its
+    // three x weights and three y weights as separate arrays.  The nine source slots of this
+    // thread's destination each have an index opaque to the compiler (an empty statement
+    // that claims to change it), so reads of neighbouring slots are not paired with each
+    // other (which needs an address register per array and pair); every read is a single
+    // load at a constant offset from its slot's index
+    constexpr int ZA = 0, ZB = 2 * T::SP, WXO = 4 * T::SP, WYO = 7 * T::SP;
+    int qs[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        qs[j] = (cx + j % 3) + T::SX * (cy + j / 3);
+        asm volatile("" : "+v"(qs[j]));
+    }
     auto step = [&](int zp, const float (&cur)[T::ITEMS][4], float (&nxt)[T::ITEMS][4]) {
         if (zp < z1) fetch(zp + 1, nxt);
         lds_barrier();  // the previous plane's weights are no longer read
@@ -887,8 +901,8 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) v
             if (item >= nitems) break;
             const int c = item / T::SP, q = item % T::SP;
             float *Wc = sw_dyn + c * T::LDS_FLOATS_PER_CH;
-            Wc[q] = cur[k][3];
             const float sh[3] = {shx[k], shy[k], shz};
+            float wv[3][3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const float mu = cur[k][d];
@@ -902,9 +916,16 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) v
                     // clamp(dv, 0, clip) in one instruction: for finite dv and clip > 0 the
                     // median of (dv, 0, clip) is min(max(dv, 0), clip) (a zero's sign may
                     // differ, which no sum of these products can observe)
-                    Wc[(1 + d * 3 + kk) * T::SP + q] = __builtin_amdgcn_fmed3f(dv, 0.0f, clip);
+                    wv[d][kk] = __builtin_amdgcn_fmed3f(dv, 0.0f, clip);
                 }
             }
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) {
+                Wc[WXO + kk * T::SP + q] = wv[0][kk];
+                Wc[WYO + kk * T::SP + q] = wv[1][kk];
+            }
+            reinterpret_cast<float2 *>(Wc + ZA)[q] = make_float2(wv[2][0], wv[2][1]);
+            reinterpret_cast<float2 *>(Wc + ZB)[q] = make_float2(wv[2][2], cur[k][3]);
         }
         lds_barrier();
 #pragma unroll
@@ -918,16 +939,18 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) v
             for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
                 for (int dx = -1; dx <= 1; dx++) {
-                    const int q = (cx + dx + 1) + T::SX * (cy + dy + 1);
-                    float base = Wc[q] * Wc[(1 + dx + 1) * T::SP + q];
-                    base = base * Wc[(4 + dy + 1) * T::SP + q];
-                    float t = base * Wc[7 * T::SP + q];  // destination z'+1 sees this plane at dz = -1
+                    const int q = qs[(dx + 1) + 3 * (dy + 1)];
+                    const float2 za = reinterpret_cast<const float2 *>(Wc + ZA)[q];  // z weights: z'+1, z'
+                    const float2 zb = reinterpret_cast<const float2 *>(Wc + ZB)[q];  // z weight z'-1, mass
+                    float base = zb.y * Wc[WXO + (dx + 1) * T::SP + q];
+                    base = base * Wc[WYO + (dy + 1) * T::SP + q];
+                    float t = base * za.x;  // destination z'+1 sees this plane at dz = -1
                     t = t * sn;
                     accN[c] = accN[c] + t;
-                    t = base * Wc[8 * T::SP + q];        // destination z', dz = 0
+                    t = base * za.y;        // destination z', dz = 0
                     t = t * sn;
                     accC[c] = accC[c] + t;
-                    t = base * Wc[9 * T::SP + q];        // destination z'-1, dz = +1
+                    t = base * zb.x;        // destination z'-1, dz = +1
                     t = t * sn;
                     accP[c] = accP[c] + t;
                 }
