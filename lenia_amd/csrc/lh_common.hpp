@@ -48,6 +48,15 @@ __host__ __device__ inline bool srt_local_world(const DevWorld &w) {
     return w.mode == 0 && !(w.osc_la || w.osc_flow || w.osc_div_flow);
 }
 
+// A world whose gradient-sweep cell needs none of the optional paths: Sobel gradients of
+// S and H (no blur), SRT or RT, no MU_VEL, V not live, F and alpha not kept, no wall clamp
+// of mu, and theta_A a power of two (alpha's division an exact multiply).  A shape class of
+// only such worlds runs the sweep compiled without those paths.
+__host__ __device__ inline bool plain_local_world(const DevWorld &w) {
+    return w.gradient == 1 && !w.blur && w.mode != 2 && !w.mu_vel && !w.v_live && !w.need_f && !w.need_alpha &&
+           w.border != 1 && w.inv_theta != 0.0f;
+}
+
 // quirk bits (same values as lenia_hip.h)
 #define LHQ_SOBEL_PERMUTE   (1u << 0)
 #define LHQ_PREALPHA_INDEX  (1u << 1)

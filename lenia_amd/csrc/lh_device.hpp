@@ -209,7 +209,7 @@ __device__ __forceinline__ void local_load(const PhaseArgs &a, const DevWorld &w
 // term tt[c] from the cell's own mu (srt reintegration.cpp:607-688, as transport_cell
 // mode 0: the same expression on the same mu value), and writes mu only when kept.
 // SRTL: compiled with that path (a launch whose shape class holds such worlds).
-template <int W, int H, int D, int CM, bool SRTL, class Grad>
+template <int W, int H, int D, int CM, bool SRTL, bool PLAIN, class Grad>
 __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w, int x, int y, int z,
                                            const LocalIn<CM, SRTL> &in, Grad grad, float (&tt)[CM]) {
     using S = Shape<W, H, D>;
@@ -219,7 +219,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
     const int nrt = (w.quirks & LHQ_RT_3CH) ? (C < 3 ? C : 3) : C;
     const float dt = w.dt, ma = w.ma, nma = -w.ma, damp = w.damp, sig = w.sigma;
     const float lim[3] = {(float)W - sig, (float)H - sig, (float)D - sig};
-    const bool wall = w.border == 1;
+    const bool wall = !PLAIN && w.border == 1;
     const bool qa = (w.quirks & LHQ_PREALPHA_INDEX) != 0;
     const long long i = x + (long long)W * (y + (long long)H * z);
     float na[3];
@@ -233,10 +233,10 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
         asm volatile("" ::: "memory");
         float nu[3];
         grad(c + 1, nu);
-        float pg = w.inv_theta != 0.0f ? in.aal[c] * w.inv_theta : in.aal[c] / w.theta_a;
+        float pg = (PLAIN || w.inv_theta != 0.0f) ? in.aal[c] * w.inv_theta : in.aal[c] / w.theta_a;
         pg = pg * pg;
         const float al = clamp_med3(pg, 0.0f, 1.0f);  // pg is a square: never -0
-        if (w.need_alpha) {
+        if (!PLAIN && w.need_alpha) {
             // ALPHA = |A - GA[c*P+i]| (lenia.cpp:661-664), GA read before this step's TRM
             const int r = qa ? c * (int)S::P + (int)i : c * S::N + (int)i;
             const int gc = r / S::N;
@@ -251,13 +251,13 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
         const unsigned iu = (unsigned)i;
         float v[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) v[d] = w.v_live ? el(V, (unsigned)((3 * c + d) * S::N) + iu) : 0.0f;
-        if (w.mu_vel) {
+        for (int d = 0; d < 3; d++) v[d] = (!PLAIN && w.v_live) ? el(V, (unsigned)((3 * c + d) * S::N) + iu) : 0.0f;
+        if (!PLAIN && w.mu_vel) {
 #pragma unroll
             for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
         }
         float *MU = a.pool + w.MU;
-        if (w.mode == 2) {
+        if (!PLAIN && w.mode == 2) {
             float vn[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) vn[d] = w.mu_vel ? v[d] : (v[d] + F[d]) * damp;
@@ -270,7 +270,7 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
             float mv[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                float m = (w.mu_vel ? v[d] : F[d]) * dt;
+                float m = ((!PLAIN && w.mu_vel) ? v[d] : F[d]) * dt;
                 m = clamp_med3(m, nma, ma);  // -ma < 0 < ma
                 m = pos[d] + m;
                 if (wall) m = clamp_med3(m, sig, lim[d]);  // 0 < sigma < n - sigma
@@ -293,15 +293,15 @@ __device__ __forceinline__ void local_cell(const PhaseArgs &a, const DevWorld &w
                 tt[c] = c < nrt ? 0.0f + wt : 0.0f;  // TT starts at 0 (srt adds to it)
             }
         }
-        if (!w.mu_vel && w.v_live) {  // (MU_VEL implies a live V)
+        if (!PLAIN && !w.mu_vel && w.v_live) {  // (MU_VEL implies a live V)
 #pragma unroll
             for (int d = 0; d < 3; d++) v[d] = (v[d] + F[d]) * damp;
         }
-        if (w.v_live) {
+        if (!PLAIN && w.v_live) {
 #pragma unroll
             for (int d = 0; d < 3; d++) el(V, (unsigned)((3 * c + d) * S::N) + iu) = v[d];
         }
-        if (w.need_f) {
+        if (!PLAIN && w.need_f) {
             float *Fo = a.pool + w.Fv;
 #pragma unroll
             for (int d = 0; d < 3; d++) el(Fo, (unsigned)((3 * c + d) * S::N) + iu) = F[d];
@@ -638,14 +638,17 @@ constexpr int LOCAL_MAXT = 256;
 // in-sweep SRT transform (32^3: 85 -> 80 VGPRs, 2.17 -> 2.03 ms on 4,096 worlds) and five
 // with it (a few registers spilled; 1,024 worlds: 32^3 797 -> 681 us, 36^3 1300 -> 1181
 // us); workgroups of three waves (24^3, 44 x 24 x 44) ran slower so (372 -> 406 us,
-// 1273 -> 1454 us) and keep what the compiler picks.
-template <int W, int H, int D, int CM, bool SRTL>
+// 1273 -> 1454 us) and keep what the compiler picks.  The plain kernels need fewer
+// registers: about 60 without the SRT transform (eight waves as compiled), six waves with it.
+template <int W, int H, int D, int CM, bool SRTL, bool PLAIN>
 constexpr int local_sweep_waves() {
-    return CM == 3 && (SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS + 63) / 64 == 4 ? (SRTL ? 5 : 6) : 1;
+    if (CM != 3 || (SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS + 63) / 64 != 4) return 1;
+    return PLAIN ? (SRTL ? 6 : 1) : (SRTL ? 5 : 6);
 }
-template <int W, int H, int D, int CM, bool SRTL>
+// PLAIN: every world of the launch is a plain_local_world (the optional paths compiled out).
+template <int W, int H, int D, int CM, bool SRTL, bool PLAIN>
 __global__ __launch_bounds__((SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS))
-__attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL>()))) void k_local_sweep(PhaseArgs a, int zseg) {
+__attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL, PLAIN>()))) void k_local_sweep(PhaseArgs a, int zseg) {
     using S = Shape<W, H, D>;
     using T = SweepCfg<W, H, D, CM, LOCAL_MAXT>;
     constexpr int F = CM + 1;  // staged fields: S, H_0 .. H_{C-1}
@@ -662,8 +665,8 @@ __attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL>()))) voi
     const int kind = w.sobel_border, faces = (w.quirks & LHQ_FACE_HALO) != 0;
     // the non-blur central-difference copy (reintegration.cpp:179-186) leaves U_S zero
     // from padded index 5P/2 on
-    const bool cdcopy = w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
-    const bool blur_src = w.gradient == 0 && w.blur;
+    const bool cdcopy = !PLAIN && w.gradient == 0 && !w.blur && (w.quirks & LHQ_CD_COPY);
+    const bool blur_src = !PLAIN && w.gradient == 0 && w.blur;
     const float *srcS = blur_src ? a.pool + w.SUMS : S_cur(a, w);
     const float *srcH = a.pool + (blur_src ? w.US : w.Hg);
     const int nitems = (C + 1) * T::SP;
@@ -741,7 +744,7 @@ __attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL>()))) voi
         const LocalIn<CM, SRTL> cur = in;
         if (z + 1 < z1) local_load<W, H, D, CM, SRTL>(a, w, col + (long long)W * H * (z + 1), in);  // next plane
         float tt[CM];
-        local_cell<W, H, D, CM, SRTL>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
+        local_cell<W, H, D, CM, SRTL, PLAIN>(a, w, tx0 + cx, ty0 + cy, z, cur, [&](int f, float (&g)[3]) {
             // the corner of this thread's 3 x 3 neighbourhood in each plane of field f, each
             // offset opaque to the compiler (an empty statement that claims to change it), so
             // that the nine reads of a plane pair up off one address register; without it
@@ -750,7 +753,7 @@ __attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL>()))) voi
             int o[3] = {r0 + f * T::SP + toff, r1 + f * T::SP + toff, r2 + f * T::SP + toff};
             asm volatile("" : "+v"(o[0]), "+v"(o[1]), "+v"(o[2]));
             const float *const pl[3] = {ring + o[0], ring + o[1], ring + o[2]};
-            if (w.gradient == 1) {
+            if (PLAIN || w.gradient == 1) {
                 if (perm) sobel_planes<T::SX, true>(pl, 0, 0, g);
                 else sobel_planes<T::SX, false>(pl, 0, 0, g);
             } else {
@@ -770,21 +773,25 @@ __attribute__((amdgpu_waves_per_eu(local_sweep_waves<W, H, D, CM, SRTL>()))) voi
 }
 
 // Launches the gradient sweep of a shape class.  arg: bit 0 the class holds SRT worlds
-// moved inside the sweep, maxC << 8, planes per workgroup << 16 (0: sweep_zseg).
-template <int W, int H, int D, int CM, bool SRTL>
+// moved inside the sweep, bit 1 every world is plain_local_world, maxC << 8, planes per
+// workgroup << 16 (0: sweep_zseg).  The plain kernels exist for three channels.
+template <int W, int H, int D, int CM, bool SRTL, bool PLAIN>
 void launch_local_sweep_cm(const PhaseArgs &a, int arg, hipStream_t st) {
     using SC = SweepCfg<W, H, D, CM, LOCAL_MAXT>;
     const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
-    hipLaunchKernelGGL((k_local_sweep<W, H, D, CM, SRTL>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), 0,
-                       st, a, zs);
+    hipLaunchKernelGGL((k_local_sweep<W, H, D, CM, SRTL, PLAIN>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS),
+                       0, st, a, zs);
 }
 template <int W, int H, int D>
 void launch_local_sweep(const PhaseArgs &a, int arg, hipStream_t st) {
-    const bool c3 = ((arg >> 8) & 0xff) <= 3, srt = arg & 1;
-    if (c3 && srt) launch_local_sweep_cm<W, H, D, 3, true>(a, arg, st);
-    else if (c3) launch_local_sweep_cm<W, H, D, 3, false>(a, arg, st);
-    else if (srt) launch_local_sweep_cm<W, H, D, LHD_MAX_C, true>(a, arg, st);
-    else launch_local_sweep_cm<W, H, D, LHD_MAX_C, false>(a, arg, st);
+    const bool c3 = ((arg >> 8) & 0xff) <= 3, srt = arg & 1, plain = arg & 2;
+    if (c3 && plain) {
+        if (srt) launch_local_sweep_cm<W, H, D, 3, true, true>(a, arg, st);
+        else launch_local_sweep_cm<W, H, D, 3, false, true>(a, arg, st);
+    } else if (c3 && srt) launch_local_sweep_cm<W, H, D, 3, true, false>(a, arg, st);
+    else if (c3) launch_local_sweep_cm<W, H, D, 3, false, false>(a, arg, st);
+    else if (srt) launch_local_sweep_cm<W, H, D, LHD_MAX_C, true, false>(a, arg, st);
+    else launch_local_sweep_cm<W, H, D, LHD_MAX_C, false, false>(a, arg, st);
 }
 
 // ------------------------------------------------------------------------------------

@@ -169,6 +169,7 @@ struct ShapeClass {
     bool any_osc = false;
     bool any_sweep = false, any_other = false;  // transport by the z-sweep / by tiles
     bool any_srt_local = false;                 // SRT worlds moved by the gradient sweep
+    bool all_plain = true;                      // every world is plain_local_world
     // schedule (lh_commit's autotuner; 0 / false = the launchers' defaults)
     int zseg_local = 0, zseg_transport = 0;
     bool tiled_transport = false;  // the tiled kernel also moves the sweep's worlds
@@ -280,7 +281,9 @@ int transport_arg(const ShapeClass &c) {
     return (c.any_osc ? 1 : 0) | (sweep ? 2 : 0) | (tiles ? 4 : 0) | (c.maxC << 8) | (c.zseg_transport << 16);
 }
 
-int local_arg(const ShapeClass &c) { return (c.any_srt_local ? 1 : 0) | (c.maxC << 8) | (c.zseg_local << 16); }
+int local_arg(const ShapeClass &c) {
+    return (c.any_srt_local ? 1 : 0) | (c.all_plain ? 2 : 0) | (c.maxC << 8) | (c.zseg_local << 16);
+}
 
 void enqueue_spectra(Engine &e) {
     for (ShapeClass &c : e.classes) {
@@ -785,6 +788,7 @@ lh_status lh_commit(void *p) {
         c.maxK = std::max(c.maxK, K);
         c.any_blur |= d.blur != 0;
         c.any_osc |= (d.osc_la || d.osc_flow || d.osc_div_flow);
+        c.all_plain = c.all_plain && plain_local_world(d);
         if (sweep_world(d)) c.any_sweep = true;
         else if (srt_local_world(d)) c.any_srt_local = true;
         else c.any_other = true;
