@@ -75,7 +75,7 @@ struct MT<_Float16> {
 
 template <int N>
 struct GC {
-    static constexpr int RB = N >= 128 ? 2 : 1;  // 16-row blocks of the matrix per wave
+    static constexpr int RB = 1;                 // 16-row blocks of the matrix per wave
     static constexpr int WAVES = N / (16 * RB);
     static constexpr int THREADS = 64 * WAVES;
     static constexpr int KS = N / 32;            // k steps of 32
