@@ -258,6 +258,39 @@ def test_batched_mixed_shapes_match_single_worlds():
         assert bad == 0, (i, w["shape"], w["cfg"]["mode"], bad, err)
 
 
+def test_plain_and_full_gradient_sweeps_in_one_class():
+    """A shape class whose worlds are all plain (Sobel, SRT or RT, no MU_VEL, V not live, no
+    wall, a power-of-two theta_A) runs the gradient sweep compiled without the other paths;
+    one world needing any of them sends its whole class through the full sweep.  Each
+    batch below holds a plain class of 24^3 and a class of 16^3 that mixes plain worlds
+    with one world of each kind that is not; every world equals the checker."""
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    plain = [WL.world((24, 24, 24), 3, 9, 6, seed=700 + i, mode=m) for i, m in enumerate(("rt", "srt", "rt"))]
+    extra = (dict(border="wall", sobel_border="wall"), dict(mu_vel=True), dict(theta_a=3.0),
+             dict(gradient="central"), dict(mode="adv"))
+    for j, over in enumerate(extra):
+        mixed = [WL.world((16, 16, 16), 3, 9, 5, seed=720 + 10 * j + i, mode=m) for i, m in enumerate(("rt", "srt"))]
+        mixed.append(WL.world((16, 16, 16), 3, 9, 5, seed=729 + 10 * j, **over))
+        ws = plain + mixed
+        eng = Engine()
+        for w in ws:
+            eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        for i, w in enumerate(ws):
+            eng.set(i, "A", w["A"])
+        eng.step(4)
+        eng.synchronize()
+        got = [eng.get(i, "A") for i in range(len(ws))]
+        eng.close()
+        for i, w in enumerate(ws):
+            ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 32)
+            ow.set(O.F_A, w["A"])
+            ow.step(4)
+            bad, err = mismatch(got[i], ow.get(O.F_A))
+            assert bad == 0, (over, i, w["shape"], w["cfg"]["mode"], bad, err)
+
+
 def test_mixed_batch_4096_hundred_steps_one_launch():
     """4096 worlds of the four mixed shapes with per-world SRT/RT, advanced 100 steps by
     one call (one captured-graph launch for all shape classes); worlds of every shape and
