@@ -555,31 +555,31 @@ inline int sweep_zseg(int nworlds, int blocks_per_world, int depth) {
 // staged per row moved (the y halo) times the lanes of whole waves per lane used, for at
 // most MAXT threads and the transport's plane -- ten floats per slot and channel -- within
 // 64 KB of LDS.
-constexpr int sweep_ty(int TX, int H, int CM, int MAXT) {
+constexpr int sweep_ty(int TX, int H, int CM, int MAXT, int LDSCAP) {
     int best = 1;
     long long bnum = 0, bden = 1;  // best cost as a fraction
     for (int ty = 1; ty <= H; ty++) {
         if (H % ty) continue;
         const int thr = TX * ty;
-        if ((ty > 1 && thr > MAXT) || CM * 10 * (TX + 2) * (ty + 2) * 4 > 65536) break;
+        if ((ty > 1 && thr > MAXT) || CM * 10 * (TX + 2) * (ty + 2) * 4 > LDSCAP) break;
         const long long num = (long long)(ty + 2) * ((thr + 63) / 64 * 64), den = (long long)ty * thr;
         if (ty == 1 || num * bden < bnum * den) { best = ty; bnum = num; bden = den; }
     }
     return best;
 }
 
-template <int W, int H, int D, int CM, int MAXT>
+template <int W, int H, int D, int CM, int MAXT, int LDSCAP = 65536>
 struct SweepCfg {
     static constexpr int TX = stencil_tx(W);
     static constexpr int XT = W / TX;
-    static constexpr int TY = sweep_ty(TX, H, CM, MAXT);
+    static constexpr int TY = sweep_ty(TX, H, CM, MAXT, LDSCAP);
     static constexpr int YT = H / TY;
     static constexpr int THREADS = TX * TY;
     static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
     static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
     static constexpr int BLOCKS = XT * YT;
     static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // transport: nine weights and the mass per slot
-    static_assert(CM * LDS_FLOATS_PER_CH * 4 <= 65536, "sweep plane exceeds 64 KB");
+    static_assert(CM * LDS_FLOATS_PER_CH * 4 <= LDSCAP, "sweep plane exceeds its LDS cap");
 };
 
 // One coordinate through the transport halo (halo_src, axis by axis): resolved index
@@ -809,11 +809,16 @@ void launch_local_sweep(const PhaseArgs &a, int arg, hipStream_t st) {
 // this cap (835 -> 766 us, the gradient sweep 1491 -> 1300 us); caps of 320 and 448 (six
 // to eight rows of 44 cells on 44 x 24 x 44, twelve rows on 36^3) ran slower.
 constexpr int TRANSPORT_MAXT = 256;
+// LDS bytes the transport plane may take per workgroup (dynamic: above 64 KB the launch
+// allows it per kernel).
+constexpr int TRANSPORT_LDS = 65536;
 template <int W, int H, int D, int CM>
-__global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) void k_transport_sweep(PhaseArgs a,
+using TransportCfg = SweepCfg<W, H, D, CM, TRANSPORT_MAXT, TRANSPORT_LDS>;
+template <int W, int H, int D, int CM>
+__global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS)) void k_transport_sweep(PhaseArgs a,
                                                                                                    int zseg) {
     using S = Shape<W, H, D>;
-    using T = SweepCfg<W, H, D, CM, TRANSPORT_MAXT>;
+    using T = TransportCfg<W, H, D, CM>;
     extern __shared__ float sw_dyn[];
     // destination planes [z0, z1) of this workgroup; source planes z0-1 .. z1
     const int nseg = (D + zseg - 1) / zseg;
@@ -985,6 +990,23 @@ __global__ __launch_bounds__((SweepCfg<W, H, D, CM, TRANSPORT_MAXT>::THREADS)) v
 #pragma unroll
             for (int d = 0; d < 4; d++) cur[k][d] = nxt[k][d];
     }
+}
+
+// Launches the transport sweep of a shape class for worlds of at most maxc channels
+// (CM = 3 or LHD_MAX_C); zarg: planes per workgroup (0: sweep_zseg).
+template <int W, int H, int D, int CM>
+void launch_transport_sweep(const PhaseArgs &a, int maxc, int zarg, hipStream_t st) {
+    using SC = TransportCfg<W, H, D, CM>;
+    const size_t lds = (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float);
+    if constexpr (CM * SC::LDS_FLOATS_PER_CH * 4 > 65536) {
+        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(k_transport_sweep<W, H, D, CM>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           CM * SC::LDS_FLOATS_PER_CH * 4);
+        (void)attr;
+    }
+    const int zs = zarg ? zarg : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
+    hipLaunchKernelGGL((k_transport_sweep<W, H, D, CM>), dim3(a.nworlds * SC::BLOCKS * ns), dim3(SC::THREADS), lds, st,
+                       a, zs);
 }
 
 // RT with dd = 1 from LDS: each staged source carries its nine per-axis overlap weights

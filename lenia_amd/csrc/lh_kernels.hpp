@@ -775,19 +775,8 @@ struct Launch {
     static void transport(const PhaseArgs &a, int arg, void *st) {
         if (arg & 2) {
             const int maxc = (arg >> 8) & 0xff;
-            if (maxc <= 3) {
-                using SC = SweepCfg<W, H, D, 3, TRANSPORT_MAXT>;
-                const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
-                hipLaunchKernelGGL((k_transport_sweep<W, H, D, 3>), dim3(a.nworlds * SC::BLOCKS * ns),
-                                   dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
-                                   (hipStream_t)st, a, zs);
-            } else {
-                using SC = SweepCfg<W, H, D, LHD_MAX_C, TRANSPORT_MAXT>;
-                const int zs = (arg >> 16) ? (arg >> 16) : sweep_zseg(a.nworlds, SC::BLOCKS, D), ns = (D + zs - 1) / zs;
-                hipLaunchKernelGGL((k_transport_sweep<W, H, D, LHD_MAX_C>), dim3(a.nworlds * SC::BLOCKS * ns),
-                                   dim3(SC::THREADS), (size_t)maxc * SC::LDS_FLOATS_PER_CH * sizeof(float),
-                                   (hipStream_t)st, a, zs);
-            }
+            if (maxc <= 3) launch_transport_sweep<W, H, D, 3>(a, maxc, arg >> 16, (hipStream_t)st);
+            else launch_transport_sweep<W, H, D, LHD_MAX_C>(a, maxc, arg >> 16, (hipStream_t)st);
         }
         if (arg & 4) {
             const dim3 grid(a.nworlds * TileCfg<W, H, D>::TILES);
