@@ -3,7 +3,7 @@
 // One thread owns a whole line of N complex values in VGPRs; every index is a
 // compile-time constant after unrolling, so there is no LDS traffic and no dynamic
 // register indexing inside a transform.  The schedule is Stockham autosort with the
-// radix plan 4,...,4,[2],odd primes ascending, twiddles W_n^e = (cos, -/+sin)(2 pi e/n)
+// radix plan 8,...,8,[4],[2],odd primes ascending, twiddles W_n^e = (cos, -/+sin)(2 pi e/n)
 // read from a host-built table (double precision rounded once to fp32), odd radices
 // as paired cosine and sine sums in index order (fft_butterfly).  Built with -ffp-contract=off, so each
 // butterfly rounds exactly as written; the CPU checker restates the same schedule.
@@ -24,7 +24,8 @@ __device__ __forceinline__ void lds_barrier() {
 
 constexpr int fft_nst(int n) {
     int m = n, st = 0;
-    while (m % 4 == 0) { m /= 4; st++; }
+    while (m % 8 == 0) { m /= 8; st++; }
+    if (m % 4 == 0) { m /= 4; st++; }
     if (m % 2 == 0) { m /= 2; st++; }
     for (int q = 3; m > 1; q += 2)
         while (m % q == 0) { m /= q; st++; }
@@ -33,7 +34,8 @@ constexpr int fft_nst(int n) {
 
 constexpr int fft_radix(int n, int s) {
     int m = n, st = 0;
-    while (m % 4 == 0) { if (st == s) return 4; m /= 4; st++; }
+    while (m % 8 == 0) { if (st == s) return 8; m /= 8; st++; }
+    if (m % 4 == 0) { if (st == s) return 4; m /= 4; st++; }
     if (m % 2 == 0) { if (st == s) return 2; m /= 2; st++; }
     for (int q = 3; m > 1; q += 2)
         while (m % q == 0) { if (st == s) return q; m /= q; st++; }
@@ -72,6 +74,26 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[R], const float2 *__re
         float2 q = make_float2(b.x - d.y, b.y + d.x);  // b + i d
         v[1] = INV ? q : p;
         v[3] = INV ? p : q;
+    } else if constexpr (R == 8) {
+        // two 4-point transforms of the even and odd inputs, the odd ones turned by
+        // W_8^k (W_8 = (1 -/+ i) c, c = cos(pi/4) from the table; W_8^2 = -/+ i exactly)
+        float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        fft_butterfly<4, N, INV>(e, tw);
+        fft_butterfly<4, N, INV>(o, tw);
+        const float c = twiddle<true>(tw, N / 8).x;
+        const float2 t1 = INV ? make_float2((o[1].x - o[1].y) * c, (o[1].x + o[1].y) * c)
+                              : make_float2((o[1].x + o[1].y) * c, (o[1].y - o[1].x) * c);
+        const float2 t2 = INV ? make_float2(-o[2].y, o[2].x) : make_float2(o[2].y, -o[2].x);
+        const float2 t3 = INV ? make_float2(-((o[3].x + o[3].y) * c), (o[3].x - o[3].y) * c)
+                              : make_float2((o[3].y - o[3].x) * c, -((o[3].x + o[3].y) * c));
+        v[0] = c_add(e[0], o[0]);
+        v[4] = c_sub(e[0], o[0]);
+        v[1] = c_add(e[1], t1);
+        v[5] = c_sub(e[1], t1);
+        v[2] = c_add(e[2], t2);
+        v[6] = c_sub(e[2], t2);
+        v[3] = c_add(e[3], t3);
+        v[7] = c_sub(e[3], t3);
     } else {
         // odd R: inputs paired as s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j} (j = 1..M,
         // M = (R-1)/2); y_0 = v_0 + s_1 + ... + s_M; for k = 1..M the cosine part

@@ -46,7 +46,7 @@ static inline cpx cmul(cpx a, cpx w) {
 }
 
 /* ------------------------------------------------------------------------------ */
-/* 1-D complex FFT: Stockham autosort, radix plan 4..4,[2],odd primes ascending.    */
+/* 1-D complex FFT: Stockham autosort, radix plan 8..8,[4],[2],odd primes ascending. */
 /* Twiddles W_n^m = (cos(2 pi m/n), -/+ sin(2 pi m/n)) computed in double, rounded  */
 /* once to REAL.  Odd radices as paired cosine and sine sums (butterfly below).     */
 /* ------------------------------------------------------------------------------ */
@@ -60,7 +60,8 @@ static const fplan *get_plan(int n) {
     fplan *f = (fplan *)calloc(1, sizeof(fplan));
     f->n = n;
     int m = n;
-    while (m % 4 == 0) { f->radix[f->nst++] = 4; m /= 4; }
+    while (m % 8 == 0) { f->radix[f->nst++] = 8; m /= 8; }
+    if (m % 4 == 0) { f->radix[f->nst++] = 4; m /= 4; }
     if (m % 2 == 0) { f->radix[f->nst++] = 2; m /= 2; }
     for (int q = 3; m > 1; q += 2)
         while (m % q == 0) { f->radix[f->nst++] = q; m /= q; }
@@ -89,6 +90,27 @@ static void butterfly(const fplan *f, cpx *v, int R, int inv) {
         p.r = b.r + d.i; p.i = b.i - d.r;
         q.r = b.r - d.i; q.i = b.i + d.r;
         if (inv) { v[1] = q; v[3] = p; } else { v[1] = p; v[3] = q; }
+    } else if (R == 8) {
+        /* two 4-point transforms of the even and odd inputs, the odd ones turned by
+         * W_8^k (W_8 = (1 -/+ i) c, c = cos(pi/4) from the table; W_8^2 = -/+ i exactly) */
+        cpx e[4], o[4], t1, t2, t3;
+        for (int r = 0; r < 4; r++) { e[r] = v[2 * r]; o[r] = v[2 * r + 1]; }
+        butterfly(f, e, 4, inv);
+        butterfly(f, o, 4, inv);
+        const real c = f->c[f->n / 8];
+        if (inv) {
+            t1.r = (o[1].r - o[1].i) * c; t1.i = (o[1].r + o[1].i) * c;
+            t2.r = -o[2].i; t2.i = o[2].r;
+            t3.r = -((o[3].r + o[3].i) * c); t3.i = (o[3].r - o[3].i) * c;
+        } else {
+            t1.r = (o[1].r + o[1].i) * c; t1.i = (o[1].i - o[1].r) * c;
+            t2.r = o[2].i; t2.i = -o[2].r;
+            t3.r = (o[3].i - o[3].r) * c; t3.i = -((o[3].r + o[3].i) * c);
+        }
+        v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
+        v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
+        v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
     } else {
         /* odd R: inputs paired as s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j} (j = 1..M,
          * M = (R-1)/2); y_0 = v_0 + s_1 + ... + s_M; for k = 1..M the cosine part
