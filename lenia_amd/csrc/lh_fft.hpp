@@ -4,9 +4,10 @@
 // compile-time constant after unrolling, so there is no LDS traffic and no dynamic
 // register indexing inside a transform.  The schedule is Stockham autosort with the
 // radix plan 8,...,8,[4],[2],odd primes ascending, twiddles W_n^e = (cos, -/+sin)(2 pi e/n)
-// read from a host-built table (double precision rounded once to fp32), odd radices
-// as paired cosine and sine sums in index order (fft_butterfly).  Built with -ffp-contract=off, so each
-// butterfly rounds exactly as written; the CPU checker restates the same schedule.
+// read from a host-built table (double precision rounded once to fp32), twiddle products
+// and odd radices (paired cosine and sine sums in index order, fft_butterfly) as fused
+// multiply-adds.  Built with -ffp-contract=off, so each butterfly rounds exactly as
+// written; the CPU checker restates the same schedule.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -44,8 +45,10 @@ constexpr int fft_radix(int n, int s) {
 
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * w with one fused multiply-add per component: re = fma(a.x, w.x, -(a.y w.y)),
+// im = fma(a.x, w.y, a.y w.x) (the checker restates it with fmaf / fma)
 __device__ __forceinline__ float2 c_mul(float2 a, float2 w) {
-    return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+    return make_float2(__builtin_fmaf(a.x, w.x, -(a.y * w.y)), __builtin_fmaf(a.x, w.y, a.y * w.x));
 }
 
 // Twiddle tables are read-only for a kernel's lifetime: read through the constant
@@ -98,8 +101,9 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[R], const float2 *__re
         // odd R: inputs paired as s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j} (j = 1..M,
         // M = (R-1)/2); y_0 = v_0 + s_1 + ... + s_M; for k = 1..M the cosine part
         // A_k = v_0 + sum_j s_j cos(2 pi jk/R) and the sine part C_k = sum_j d_j sin(2 pi jk/R),
-        // both summed in j order, give y_k = A_k -/+ i C_k, y_{R-k} = A_k +/- i C_k (upper
-        // sign forward): about a quarter of a direct DFT's products
+        // both summed in j order by fused multiply-adds, give y_k = A_k -/+ i C_k,
+        // y_{R-k} = A_k +/- i C_k (upper sign forward): about a quarter of a direct DFT's
+        // products
         constexpr int M = (R - 1) / 2;
         float2 y[R], sv[M + 1], dv[M + 1];
 #pragma unroll
@@ -116,14 +120,14 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[R], const float2 *__re
 #pragma unroll
             for (int j = 1; j <= M; j++) {
                 const float2 w = twiddle<true>(tw, ((j * k) % R) * (N / R));  // (cos, sin)
-                ar = ar + sv[j].x * w.x;
-                ai = ai + sv[j].y * w.x;
+                ar = __builtin_fmaf(sv[j].x, w.x, ar);
+                ai = __builtin_fmaf(sv[j].y, w.x, ai);
                 if (j == 1) {
                     cr = dv[j].x * w.y;
                     ci = dv[j].y * w.y;
                 } else {
-                    cr = cr + dv[j].x * w.y;
-                    ci = ci + dv[j].y * w.y;
+                    cr = __builtin_fmaf(dv[j].x, w.y, cr);
+                    ci = __builtin_fmaf(dv[j].y, w.y, ci);
                 }
             }
             const float2 p = make_float2(ar + ci, ai - cr), q = make_float2(ar - ci, ai + cr);  // A -/+ iC

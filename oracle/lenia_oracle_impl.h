@@ -38,17 +38,23 @@ static inline int iemod(int a, int b) { int r = a % b; return r < 0 ? r + b : r;
 
 static inline cpx cadd(cpx a, cpx b) { cpx c; c.r = a.r + b.r; c.i = a.i + b.i; return c; }
 static inline cpx csub(cpx a, cpx b) { cpx c; c.r = a.r - b.r; c.i = a.i - b.i; return c; }
+/* a * b + c rounded once (fmaf / fma) */
+static inline real rfma(real a, real b, real c) {
+    return sizeof(real) == sizeof(float) ? (real)fmaf((float)a, (float)b, (float)c) : (real)fma((double)a, (double)b, (double)c);
+}
+/* twiddle product with one fused multiply-add per component */
 static inline cpx cmul(cpx a, cpx w) {
     cpx c;
-    c.r = a.r * w.r - a.i * w.i;
-    c.i = a.r * w.i + a.i * w.r;
+    c.r = rfma(a.r, w.r, -(a.i * w.i));
+    c.i = rfma(a.r, w.i, a.i * w.r);
     return c;
 }
 
 /* ------------------------------------------------------------------------------ */
 /* 1-D complex FFT: Stockham autosort, radix plan 8..8,[4],[2],odd primes ascending. */
 /* Twiddles W_n^m = (cos(2 pi m/n), -/+ sin(2 pi m/n)) computed in double, rounded  */
-/* once to REAL.  Odd radices as paired cosine and sine sums (butterfly below).     */
+/* once to REAL.  Twiddle products and the paired cosine and sine sums of odd radices */
+/* (butterfly below) as fused multiply-adds.                                         */
 /* ------------------------------------------------------------------------------ */
 #define MAXN 256
 #define MAXR 32
@@ -127,10 +133,10 @@ static void butterfly(const fplan *f, cpx *v, int R, int inv) {
             for (int j = 1; j <= M; j++) {
                 const int e = ((j * k) % R) * step;
                 const real c = f->c[e], sn = f->s[e];
-                ar = ar + sv[j].r * c;
-                ai = ai + sv[j].i * c;
+                ar = rfma(sv[j].r, c, ar);
+                ai = rfma(sv[j].i, c, ai);
                 if (j == 1) { cr = dv[j].r * sn; ci = dv[j].i * sn; }
-                else { cr = cr + dv[j].r * sn; ci = ci + dv[j].i * sn; }
+                else { cr = rfma(dv[j].r, sn, cr); ci = rfma(dv[j].i, sn, ci); }
             }
             cpx p, q; /* p = A - iC, q = A + iC */
             p.r = ar + ci; p.i = ai - cr;
