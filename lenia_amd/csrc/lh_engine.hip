@@ -464,8 +464,13 @@ float time_phase(Engine &e, launch_fn fn, const PhaseArgs &a, int arg, int reps)
 
 // Schedule autotuner: per shape class, times the gradient sweep at several segment
 // lengths and the transport as z-sweep (same lengths) or as tiles, on the class's own
-// worlds, and keeps the fastest.  Runs on the freshly zeroed pool (every kernel is
-// defined on zeros); the caller clears the pool afterwards.
+// worlds, and keeps the fastest.  Candidates are tried from the whole depth down; a
+// shorter segment (more workgroups, more halo planes staged) or the tiled transport
+// replaces the current pick only when faster by more than the timing spread
+// (kTuneMargin), so that two near-equal candidates do not flip between runs.  Runs on
+// the freshly zeroed pool (every kernel is defined on zeros); the caller clears the pool
+// afterwards.
+constexpr float kTuneMargin = 0.97f;
 lh_status autotune(Engine &e) {
     for (ShapeClass &c : e.classes) {
         PhaseArgs a = args_for(e, c, 0);
@@ -473,13 +478,13 @@ lh_status autotune(Engine &e) {
         std::vector<int> cands = {D};
         for (int zs : {D / 2, D / 4, 8, 4, 2, 1})
             if (zs >= 1 && std::find(cands.begin(), cands.end(), zs) == cands.end()) cands.push_back(zs);
-        const int reps = c.ids.size() >= 256 ? 2 : 5;
+        const int reps = c.ids.size() >= 256 ? 3 : 5;
         float best = 1e30f;
         int pick_local = D;
         for (int zs : cands) {
             c.zseg_local = zs;
             const float t = time_phase(e, c.ops->local, a, local_arg(c), reps);
-            if (t < best) { best = t; pick_local = zs; }
+            if (t < best * kTuneMargin) { best = t; pick_local = zs; }
         }
         c.zseg_local = pick_local;
         if (c.ops->growth_fused && !c.gops) {
@@ -497,12 +502,12 @@ lh_status autotune(Engine &e) {
                 c.zseg_transport = zs;
                 c.tiled_transport = false;
                 const float t = time_phase(e, c.ops->transport, a, transport_arg(c), reps);
-                if (t < best) { best = t; pick = zs; tiled = false; }
+                if (t < best * kTuneMargin) { best = t; pick = zs; tiled = false; }
             }
             c.tiled_transport = true;
             c.zseg_transport = 0;
             const float t = time_phase(e, c.ops->transport, a, transport_arg(c), reps);
-            if (t < best) { best = t; tiled = true; pick = 0; }
+            if (t < best * kTuneMargin) { best = t; tiled = true; pick = 0; }
             c.tiled_transport = tiled;
             c.zseg_transport = pick;
         }
