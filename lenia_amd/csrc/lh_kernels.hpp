@@ -44,7 +44,7 @@ template <int W, int H, int D, int NP>
 __global__ __launch_bounds__((fwd_threads<W, H, NP>())) void k_fwd_xy(PhaseArgs a, int arg) {
     using S = Shape<W, H, D>;
     constexpr int Wh = S::Wh, T = fwd_threads<W, H, NP>();
-    __shared__ float2 sspec[NP * H * Wh];
+    __shared__ __attribute__((aligned(16))) float2 sspec[NP * H * Wh];
     const int src = arg & 0xFF, nfmax = arg >> 8;
     const int groups = D / NP;
     const int per_world = nfmax * groups;
@@ -100,10 +100,15 @@ __global__ __launch_bounds__((fwd_threads<W, H, NP>())) void k_fwd_xy(PhaseArgs 
 #pragma unroll
         for (int y = 0; y < H; y++) line[y] = sspec[(p * H + y) * Wh + kx];
         fft_line<H, false>(line, a.twH);
-        float2 *o = out + (long long)(z0 + p) * H * Wh + kx;
 #pragma unroll
-        for (int y = 0; y < H; y++) o[y * Wh] = line[y];
+        for (int y = 0; y < H; y++) sspec[(p * H + y) * Wh + kx] = line[y];
     }
+    lds_barrier();
+    // the workgroup's planes are contiguous in the output (and H is even, so a plane is a
+    // whole number of 16-byte words): written out as 16-byte words in order
+    const float4 *s4 = reinterpret_cast<const float4 *>(sspec);
+    float4 *o4 = reinterpret_cast<float4 *>(out + (long long)z0 * H * Wh);
+    for (int i = tid; i < NP * H * Wh / 2; i += T) o4[i] = s4[i];
 }
 
 // Below this many columns in a batch (256 CUs x 256 lanes), spec_z runs one thread per
