@@ -227,6 +227,12 @@ struct Engine {
     };
     std::vector<StepGraph> graphs;
     uint64_t graph_clock = 0;
+    // with several shape classes, a captured step sequence holds one branch per class
+    // (classes share no data a step writes): its streams and the fork / join events,
+    // created on the first such capture
+    std::vector<hipStream_t> branch;
+    hipEvent_t fork_ev = nullptr;
+    std::vector<hipEvent_t> join_ev;
     std::vector<std::string> prof_names;
     std::map<std::string, Prof> prof;
 };
@@ -246,22 +252,22 @@ PhaseArgs args_for(Engine &e, ShapeClass &c, int parity, float *pool2 = nullptr)
     return a;
 }
 
-// Launches one phase; in profile mode brackets it with events on the engine stream.
-void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg) {
+// Launches one phase on stream st; in profile mode brackets it with events.
+void run(Engine &e, const char *name, launch_fn fn, const PhaseArgs &a, int arg, hipStream_t st) {
     if (!e.opt_profile) {
-        fn(a, arg, e.stream);
+        fn(a, arg, st);
         return;
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     float ms = -1.0f;
     if (hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess &&
-        hipEventRecord(t0, e.stream) == hipSuccess) {
-        fn(a, arg, e.stream);
-        if (hipEventRecord(t1, e.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+        hipEventRecord(t0, st) == hipSuccess) {
+        fn(a, arg, st);
+        if (hipEventRecord(t1, st) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
             hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
             ms = -1.0f;
     } else {
-        fn(a, arg, e.stream);  // launched untimed
+        fn(a, arg, st);  // launched untimed
     }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
@@ -289,41 +295,46 @@ void enqueue_spectra(Engine &e) {
     for (ShapeClass &c : e.classes) {
         if (c.gops) continue;  // the matrix-core path transforms A at the start of each step
         PhaseArgs a = args_for(e, c, e.parity);
-        run(e, "fwd_xy", c.ops->fwd_xy, a, SRC_A | (c.maxC << 8));
+        run(e, "fwd_xy", c.ops->fwd_xy, a, SRC_A | (c.maxC << 8), e.stream);
+    }
+}
+
+// One step of one shape class, on stream st.
+void enqueue_class_step(Engine &e, ShapeClass &c, int parity, hipStream_t st) {
+    {
+        PhaseArgs a = args_for(e, c, parity);
+        if (c.gops) {
+            run(e, "dft_fx", c.gops->fx, a, c.maxC, st);
+            run(e, "dft_fy", c.gops->fy, a, c.maxC, st);
+            run(e, "dft_fz", c.gops->fz, a, c.maxC, st);
+            run(e, "dft_iz", c.gops->iz, a, c.maxK, st);
+            run(e, "dft_iy", c.gops->iy, a, c.maxK, st);
+            run(e, "dft_ix_growth", c.gops->ix, a, 0, st);
+            run(e, "local", c.ops->local, a, local_arg(c), st);
+            if (c.any_sweep || c.any_other) run(e, "transport", c.ops->transport, a, transport_arg(c), st);
+            return;
+        }
+        if (c.fused_growth) {
+            run(e, "growth_fused", c.ops->growth_fused, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8), st);
+        } else {
+            run(e, "spec_z", c.ops->spec_z, a, 0, st);
+            run(e, "inv_growth", c.ops->inv_growth, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8), st);
+        }
+        if (c.any_blur) {
+            run(e, "fwd_xy_blur", c.ops->fwd_xy, a, SRC_HBLUR | ((c.maxC + 1) << 8), st);
+            run(e, "spec_z_blur", c.ops->spec_z, a, 1, st);
+            run(e, "inv_blur", c.ops->inv_blur, a, 0, st);
+        }
+        run(e, "local", c.ops->local, a, local_arg(c), st);
+        // (a class of SRT worlds only was moved by the gradient sweep)
+        if (c.any_sweep || c.any_other) run(e, "transport", c.ops->transport, a, transport_arg(c), st);
+        PhaseArgs an = args_for(e, c, parity ^ 1);
+        run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8), st);
     }
 }
 
 void enqueue_step(Engine &e, int parity) {
-    for (ShapeClass &c : e.classes) {
-        PhaseArgs a = args_for(e, c, parity);
-        if (c.gops) {
-            run(e, "dft_fx", c.gops->fx, a, c.maxC);
-            run(e, "dft_fy", c.gops->fy, a, c.maxC);
-            run(e, "dft_fz", c.gops->fz, a, c.maxC);
-            run(e, "dft_iz", c.gops->iz, a, c.maxK);
-            run(e, "dft_iy", c.gops->iy, a, c.maxK);
-            run(e, "dft_ix_growth", c.gops->ix, a, 0);
-            run(e, "local", c.ops->local, a, local_arg(c));
-            if (c.any_sweep || c.any_other) run(e, "transport", c.ops->transport, a, transport_arg(c));
-            continue;
-        }
-        if (c.fused_growth) {
-            run(e, "growth_fused", c.ops->growth_fused, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8));
-        } else {
-            run(e, "spec_z", c.ops->spec_z, a, 0);
-            run(e, "inv_growth", c.ops->inv_growth, a, (e.opt_keep_usum ? 1 : 0) | (c.maxC << 8));
-        }
-        if (c.any_blur) {
-            run(e, "fwd_xy_blur", c.ops->fwd_xy, a, SRC_HBLUR | ((c.maxC + 1) << 8));
-            run(e, "spec_z_blur", c.ops->spec_z, a, 1);
-            run(e, "inv_blur", c.ops->inv_blur, a, 0);
-        }
-        run(e, "local", c.ops->local, a, local_arg(c));
-        // (a class of SRT worlds only was moved by the gradient sweep)
-        if (c.any_sweep || c.any_other) run(e, "transport", c.ops->transport, a, transport_arg(c));
-        PhaseArgs an = args_for(e, c, parity ^ 1);
-        run(e, "fwd_xy", c.ops->fwd_xy, an, SRC_A | (c.maxC << 8));
-    }
+    for (ShapeClass &c : e.classes) enqueue_class_step(e, c, parity, e.stream);
 }
 
 lh_status validate(const lh_shape &s, const lh_params &p, const lh_kernel *k) {
@@ -589,8 +600,9 @@ constexpr int kMaxGraphSteps = 256;
 constexpr size_t kMaxGraphs = 6;
 
 // The graph advancing every world by `steps` steps from A-buffer parity `parity`,
-// captured on first use (the kernels of all shape classes, step after step, on the
-// engine stream).
+// captured on first use: the kernels of a shape class step after step, each class's
+// chain a branch of its own (forked from and joined into the engine stream) when there
+// are several, so the classes' kernels may run side by side.
 lh_status step_graph(Engine &e, int parity, int steps, hipGraphExec_t *out) {
     for (Engine::StepGraph &g : e.graphs)
         if (g.parity == parity && g.steps == steps) {
@@ -605,10 +617,36 @@ lh_status step_graph(Engine &e, int parity, int steps, hipGraphExec_t *out) {
         (void)hipGraphExecDestroy(lru->exec);
         e.graphs.erase(lru);
     }
+    const size_t nc = e.classes.size();
+    if (nc > 1 && e.branch.size() < nc) {
+        if (!e.fork_ev) HIPCHK(hipEventCreateWithFlags(&e.fork_ev, hipEventDisableTiming));
+        while (e.branch.size() < nc) {
+            hipStream_t st = nullptr;
+            hipEvent_t ev = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            e.branch.push_back(st);
+            HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            e.join_ev.push_back(ev);
+        }
+    }
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < steps; s++) enqueue_step(e, parity ^ (s & 1));
-    HIPCHK(hipStreamEndCapture(e.stream, &g));
+    hipError_t ce = hipSuccess;
+    if (nc > 1) {
+        // one branch per shape class, forked from and joined back into the engine stream
+        ce = hipEventRecord(e.fork_ev, e.stream);
+        for (size_t i = 0; i < nc && ce == hipSuccess; i++) {
+            ce = hipStreamWaitEvent(e.branch[i], e.fork_ev, 0);
+            for (int s = 0; s < steps && ce == hipSuccess; s++) enqueue_class_step(e, e.classes[i], parity ^ (s & 1), e.branch[i]);
+            if (ce == hipSuccess) ce = hipEventRecord(e.join_ev[i], e.branch[i]);
+            if (ce == hipSuccess) ce = hipStreamWaitEvent(e.stream, e.join_ev[i], 0);
+        }
+    } else {
+        for (int s = 0; s < steps; s++) enqueue_step(e, parity ^ (s & 1));
+    }
+    const hipError_t ee = hipStreamEndCapture(e.stream, &g);
+    HIPCHK(ce);
+    HIPCHK(ee);
     hipGraphExec_t ex = nullptr;
     const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -665,6 +703,9 @@ lh_status lh_destroy(void *p) {
         if (sn.done) (void)hipEventDestroy(sn.done);
     }
     if (e->rb_stream) (void)hipStreamDestroy(e->rb_stream);
+    for (hipStream_t st : e->branch) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : e->join_ev) (void)hipEventDestroy(ev);
+    if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return LH_OK;
