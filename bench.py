@@ -10,7 +10,9 @@ next to the same engine driving a single world.  One JSON line on stdout.
 
 "roofline": the whole step's compulsory bytes (8 C N + 4 K Noct per world-step) over the
 dominant kernel's average launch time (HIP events on the engine's stream), against the
-8 TB/s HBM peak; "kernel_local" restates it with that kernel's own compulsory bytes.
+8 TB/s HBM peak; "kernel_local" restates it with that kernel's own compulsory bytes.  A
+batch of at least 2048 worlds of one shape runs as two halves (two launches of each
+phase per step, each stepping half the worlds), so a launch's bytes are half the step's.
 "traffic": HBM bytes per launch of that kernel from two rocprofv3 --pmc child runs of
 this script (FETCH_SIZE, WRITE_SIZE), started before this process touches the GPU and
 scaled by what the counters read for a plain copy of known size.
@@ -596,27 +598,34 @@ def main():
         peng.synchronize()
         prof = peng.profile()
         peng.close()
-        kernels = {k: {"launches": n, "avg_us": ms * 1e3 / n} for k, (n, ms) in prof.items()}
+        kernels = {k: {"launches": n, "avg_us": ms * 1e3 / n, "us_per_step": ms * 1e3 / args.steps}
+                   for k, (n, ms) in prof.items()}
         dom = max(prof, key=lambda k: prof[k][1])
         n, ms = prof[dom]
         avg_s = ms / n * 1e-3
+        # a batch of at least 2048 worlds runs as two halves: each phase launches once per
+        # half, each launch over half the worlds
+        lps = n / args.steps
         # the whole step's compulsory bytes (8 C N + 4 K Noct per world-step) over the
-        # dominant kernel's launch time
+        # dominant kernel's time; per launch: the share of the worlds the launch steps
         step_bytes = RF.step_bytes((32, 32, 32), 3, 9) * args.worlds
-        achieved = step_bytes / avg_s / 1e9
-        kb = RF.kernel_bytes(dom, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds
+        launch_bytes = step_bytes / lps
+        achieved = launch_bytes / avg_s / 1e9
+        kb = RF.kernel_bytes(dom, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds / lps
         line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                            "traffic": traffic.get(dom) if traffic else None,
+                            "traffic": traffic[dom] / lps if traffic and dom in traffic else None,
                             "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this bench, "
                                                "scaled by the counters of a plain copy"
                                                if traffic else traffic_notes),
-                            "algorithmic_bytes_per_launch": step_bytes, "avg_launch_us": avg_s * 1e6,
+                            "algorithmic_bytes_per_launch": launch_bytes, "avg_launch_us": avg_s * 1e6,
+                            "launches_per_step": lps,
                             "kernel_local": {"bytes_per_launch": kb, "achieved": kb / avg_s / 1e9,
                                              "frac": kb / avg_s / 1e9 / HBM_PEAK_GBS}}
         if traffic:
-            line["traffic_per_launch"] = {k: {"bytes": v, "alg_bytes": RF.kernel_bytes(
-                k, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds if k in RF.KNOWN else None}
+            per_step = {k: prof[k][0] / args.steps if k in prof else 1.0 for k in traffic}
+            line["traffic_per_launch"] = {k: {"bytes": v / per_step[k], "alg_bytes": RF.kernel_bytes(
+                k, (32, 32, 32), 3, 9, mode="rt", v_live=False) * args.worlds / per_step[k] if k in RF.KNOWN else None}
                 for k, v in traffic.items()}
             line["traffic_per_launch"]["calibration"] = traffic_notes
         step_flops = RF.step_flops((32, 32, 32), 3, 9) * args.worlds

@@ -482,6 +482,9 @@ float time_phase(Engine &e, launch_fn fn, const PhaseArgs &a, int arg, int reps)
 // the freshly zeroed pool (every kernel is defined on zeros); the caller clears the pool
 // afterwards.
 constexpr float kTuneMargin = 0.97f;
+// Worlds of one shape class from which it runs as two halves (graph branches that
+// overlap): 4096 x 32^3 7.72 -> 7.58 ms per step (four quarters: 7.62).
+constexpr size_t kSplitWorlds = 2048;
 lh_status autotune(Engine &e) {
     for (ShapeClass &c : e.classes) {
         PhaseArgs a = args_for(e, c, 0);
@@ -799,6 +802,18 @@ lh_status lh_commit(void *p) {
         }
         hw.cls = ci;
         e->classes[ci].ids.push_back(i);
+    }
+    // a class of at least kSplitWorlds worlds runs as two halves, two branches of the
+    // step graph whose kernels overlap (each half still fills the GPU)
+    const size_t n0 = e->classes.size();
+    for (size_t j = 0; j < n0; j++) {
+        if (e->classes[j].ids.size() < kSplitWorlds) continue;
+        ShapeClass half = e->classes[j];
+        const size_t h = e->classes[j].ids.size() / 2;
+        half.ids.assign(e->classes[j].ids.begin() + h, e->classes[j].ids.end());
+        e->classes[j].ids.resize(h);
+        e->classes.push_back(half);
+        for (int i : e->classes.back().ids) e->worlds[i].cls = (int)e->classes.size() - 1;
     }
     // pool layout
     int64_t off = 0;

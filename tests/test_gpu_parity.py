@@ -383,6 +383,35 @@ def test_step_graph_cache_evicts_and_stays_exact():
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_large_class_runs_as_two_halves():
+    """A class of at least 2048 worlds runs as two halves (two branches of the step
+    graph): worlds from both halves, mixed SRT and RT, equal the checker after a graph
+    launch of several steps, and the graph equals eager stepping bit for bit."""
+    from lenia_amd.engine import Engine
+    from oracle import oracle as O
+    ws = [WL.world((8, 8, 8), 3, 9, 3, seed=900 + i, mode="srt" if i % 3 == 0 else "rt") for i in range(2052)]
+    outs = []
+    for graph in (True, False):
+        eng = Engine(graph=graph)
+        for w in ws:
+            eng.add_world(w["shape"], 3, w["kernels"], w["cfg"])
+        eng.commit()
+        for i, w in enumerate(ws):
+            eng.set(i, "A", w["A"])
+        eng.step(5)
+        eng.synchronize()
+        outs.append({i: eng.get(i, "A") for i in (0, 1, 1025, 1026, 2050, 2051)})
+        eng.close()
+    for i, a in outs[0].items():
+        assert np.array_equal(a.view(np.uint32), outs[1][i].view(np.uint32)), i
+        w = ws[i]
+        ow = O.World(w["shape"], 3, w["kernels"], w["cfg"], 32)
+        ow.set(O.F_A, w["A"])
+        ow.step(5)
+        bad, err = mismatch(a, ow.get(O.F_A))
+        assert bad == 0, (i, w["cfg"]["mode"], bad, err)
+
+
 def test_srt_mu_needs_keep_mu():
     """SRT worlds move inside the gradient sweep and do not store mu unless asked."""
     from lenia_amd import _lib as L
