@@ -553,8 +553,8 @@ inline int sweep_zseg(int nworlds, int blocks_per_world, int depth) {
 // a TX x TY column of cells through the whole depth and stages one z-plane at a time.
 // Rows per sweep workgroup: the divisor of H with the least overhead, (TY + 2) / TY rows
 // staged per row moved (the y halo) times the lanes of whole waves per lane used, for at
-// most MAXT threads and the transport's plane -- ten floats per slot and channel -- within
-// 64 KB of LDS.
+// most MAXT threads and the transport's plane within 64 KB of LDS, the plane counted at
+// ten floats per slot and channel (it holds nine; the rows stay those chosen for ten).
 constexpr int sweep_ty(int TX, int H, int CM, int MAXT, int LDSCAP) {
     int best = 1;
     long long bnum = 0, bden = 1;  // best cost as a fraction
@@ -578,7 +578,8 @@ struct SweepCfg {
     static constexpr int SX = TX + 2, SY = TY + 2, SP = SX * SY;
     static constexpr int ITEMS = (SP * CM + THREADS - 1) / THREADS;  // staged (slot, channel) per thread
     static constexpr int BLOCKS = XT * YT;
-    static constexpr int LDS_FLOATS_PER_CH = 10 * SP;  // transport: nine weights and the mass per slot
+    // transport: nine weights per slot (the x weights times the mass), float2-aligned
+    static constexpr int LDS_FLOATS_PER_CH = 9 * SP + (SP & 1);
     static_assert(CM * LDS_FLOATS_PER_CH * 4 <= LDSCAP, "sweep plane exceeds its LDS cap");
 };
 
@@ -802,7 +803,8 @@ void launch_local_sweep(const PhaseArgs &a, int arg, hipStream_t st) {
 // it touches: z'+1 (dz = -1), z' (dz = 0), z'-1 (dz = +1).  A destination therefore
 // receives its 27 terms in the reference's order (dz, dy, dx), and plane z'-1 is
 // complete after step z', when its TRM runs.  The product arr*wx*wy of a source is the
-// same for the three destinations and is formed once.
+// same for the three destinations and is formed once, its arr*wx once per source when the
+// plane is staged.
 // ------------------------------------------------------------------------------------
 // CM: channels moved, at most (3 or LHD_MAX_C).  Thread cap of a workgroup (sweep_ty picks
 // its rows): four waves.  On batches of 1,024 worlds, 36^3 moved from 4 rows to 6 under
@@ -814,9 +816,14 @@ constexpr int TRANSPORT_MAXT = 256;
 constexpr int TRANSPORT_LDS = 65536;
 template <int W, int H, int D, int CM>
 using TransportCfg = SweepCfg<W, H, D, CM, TRANSPORT_MAXT, TRANSPORT_LDS>;
+// Three channels are compiled for four waves per SIMD (at most 128 VGPRs): left to itself
+// the compiler gives 24^3, 36^3 and 44 x 24 x 44 152 VGPRs (three waves) where one spilled
+// register keeps four (1,024 worlds: 44 x 24 x 44 746 -> 687 us, 24^3 239 -> 235 us, the
+// others unchanged); four channels (168 VGPRs) would spill 23 or more and keep the
+// compiler's choice.
 template <int W, int H, int D, int CM>
-__global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS)) void k_transport_sweep(PhaseArgs a,
-                                                                                                   int zseg) {
+__global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS))
+__attribute__((amdgpu_waves_per_eu(CM == 3 ? 4 : 1))) void k_transport_sweep(PhaseArgs a, int zseg) {
     using S = Shape<W, H, D>;
     using T = TransportCfg<W, H, D, CM>;
     extern __shared__ float sw_dyn[];
@@ -888,14 +895,16 @@ __global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS)) void k_transp
         accN[c] = 0.0f;
     }
     TrmIn<CM> pre;  // TRM inputs of destination plane z'-1, loaded during step z'-1
-    // LDS per channel (10 * SP floats): the z weights of a source toward destinations
-    // z'+1 and z' as one float2, its z weight toward z'-1 and its mass as another, then its
-    // three x weights and three y weights as separate arrays.  The nine source slots of this
+    // LDS per channel (9 * SP floats): the z weights of a source toward destinations z'+1
+    // and z' as one float2, its z weight toward z'-1, then its three x weights each
+    // multiplied by its mass (arr * wx, the first product of the reference's
+    // arr * wx * wy * wz * sn, formed once per source instead of once per destination) and
+    // its three y weights, as separate arrays.  The nine source slots of this
     // thread's destination each have an index opaque to the compiler (an empty statement
     // that claims to change it), so reads of neighbouring slots are not paired with each
     // other (which needs an address register per array and pair); every read is a single
     // load at a constant offset from its slot's index
-    constexpr int ZA = 0, ZB = 2 * T::SP, WXO = 4 * T::SP, WYO = 7 * T::SP;
+    constexpr int ZA = 0, ZB = 2 * T::SP, WXO = 3 * T::SP, WYO = 6 * T::SP;
     int qs[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) {
@@ -933,11 +942,11 @@ __global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS)) void k_transp
             }
 #pragma unroll
             for (int kk = 0; kk < 3; kk++) {
-                Wc[WXO + kk * T::SP + q] = wv[0][kk];
+                Wc[WXO + kk * T::SP + q] = cur[k][3] * wv[0][kk];  // mass * x weight
                 Wc[WYO + kk * T::SP + q] = wv[1][kk];
             }
             reinterpret_cast<float2 *>(Wc + ZA)[q] = make_float2(wv[2][0], wv[2][1]);
-            reinterpret_cast<float2 *>(Wc + ZB)[q] = make_float2(wv[2][2], cur[k][3]);
+            Wc[ZB + q] = wv[2][2];
         }
         lds_barrier();
 #pragma unroll
@@ -953,8 +962,8 @@ __global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS)) void k_transp
                 for (int dx = -1; dx <= 1; dx++) {
                     const int q = qs[(dx + 1) + 3 * (dy + 1)];
                     const float2 za = reinterpret_cast<const float2 *>(Wc + ZA)[q];  // z weights: z'+1, z'
-                    const float2 zb = reinterpret_cast<const float2 *>(Wc + ZB)[q];  // z weight z'-1, mass
-                    float base = zb.y * Wc[WXO + (dx + 1) * T::SP + q];
+                    const float zb = Wc[ZB + q];  // z weight z'-1
+                    float base = Wc[WXO + (dx + 1) * T::SP + q];  // mass * x weight
                     base = base * Wc[WYO + (dy + 1) * T::SP + q];
                     float t = base * za.x;  // destination z'+1 sees this plane at dz = -1
                     t = t * sn;
@@ -962,7 +971,7 @@ __global__ __launch_bounds__((TransportCfg<W, H, D, CM>::THREADS)) void k_transp
                     t = base * za.y;        // destination z', dz = 0
                     t = t * sn;
                     accC[c] = accC[c] + t;
-                    t = base * zb.x;        // destination z'-1, dz = +1
+                    t = base * zb;          // destination z'-1, dz = +1
                     t = t * sn;
                     accP[c] = accP[c] + t;
                 }
