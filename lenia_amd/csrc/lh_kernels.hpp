@@ -70,27 +70,48 @@ __global__ __launch_bounds__((fwd_threads<W, H, NP>())) void k_fwd_xy(PhaseArgs 
     }
     if (f >= nf) return;
     const int tid = threadIdx.x;
-    // x: rows y, y+1 straight from HBM (W is a multiple of 4: 16-byte row starts)
-    for (int t = tid; t < NP * (H / 2); t += T) {
-        const int p = t / (H / 2), y = 2 * (t % (H / 2));
-        const float4 *r0 = reinterpret_cast<const float4 *>(in + (long long)(z0 + p) * S::PL + y * W);
-        const float4 *r1 = r0 + W / 4;
-        float2 line[W];
+    // the workgroup's NP planes (contiguous in HBM) staged in LDS by 16-byte loads of
+    // consecutive words, rows at a pitch of W + 1 floats, in the space the spectrum takes
+    // after them (NP*H*(W+1) <= NP*H*(W+2) floats)
+    {
+        constexpr int RP = W + 1, N4 = NP * H * W / 4, J4 = (N4 + T - 1) / T;
+        const float4 *src4 = reinterpret_cast<const float4 *>(in + (long long)z0 * S::PL);
+        float4 u[J4];
 #pragma unroll
-        for (int x = 0; x < W / 4; x++) {
-            const float4 u = r0[x], v = r1[x];
-            line[4 * x + 0] = make_float2(u.x, v.x);
-            line[4 * x + 1] = make_float2(u.y, v.y);
-            line[4 * x + 2] = make_float2(u.z, v.z);
-            line[4 * x + 3] = make_float2(u.w, v.w);
+        for (int j = 0; j < J4; j++) u[j] = tid + j * T < N4 ? src4[tid + j * T] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float *stg = reinterpret_cast<float *>(sspec);
+#pragma unroll
+        for (int j = 0; j < J4; j++) {
+            const int i = tid + j * T;
+            if (i < N4) {
+                float *d = stg + (4 * i / W) * RP + (4 * i) % W;
+                d[0] = u[j].x; d[1] = u[j].y; d[2] = u[j].z; d[3] = u[j].w;
+            }
         }
-        fft_line<W, false>(line, a.twW);
-        float2 *o0 = sspec + (p * H + y) * Wh, *o1 = o0 + Wh;
+    }
+    lds_barrier();
+    // x: rows y, y+1 (one row pair per thread at most: T >= NP * H / 2), read from the
+    // staged planes before any spectrum row is written over them
+    {
+        constexpr int RP = W + 1;
+        const bool rows = tid < NP * (H / 2);
+        const int p = tid / (H / 2), y = 2 * (tid % (H / 2));
+        float2 line[W];
+        if (rows) {
+            const float *r0 = reinterpret_cast<const float *>(sspec) + (p * H + y) * RP, *r1 = r0 + RP;
 #pragma unroll
-        for (int k = 0; k < Wh; k++) {
-            const float2 Z = line[k], P = line[(W - k) % W];
-            o0[k] = make_float2((Z.x + P.x) * 0.5f, (Z.y - P.y) * 0.5f);
-            o1[k] = make_float2((Z.y + P.y) * 0.5f, (P.x - Z.x) * 0.5f);
+            for (int x = 0; x < W; x++) line[x] = make_float2(r0[x], r1[x]);
+        }
+        lds_barrier();
+        if (rows) {
+            fft_line<W, false>(line, a.twW);
+            float2 *o0 = sspec + (p * H + y) * Wh, *o1 = o0 + Wh;
+#pragma unroll
+            for (int k = 0; k < Wh; k++) {
+                const float2 Z = line[k], P = line[(W - k) % W];
+                o0[k] = make_float2((Z.x + P.x) * 0.5f, (Z.y - P.y) * 0.5f);
+                o1[k] = make_float2((Z.y + P.y) * 0.5f, (P.x - Z.x) * 0.5f);
+            }
         }
     }
     lds_barrier();
