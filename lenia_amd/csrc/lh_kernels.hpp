@@ -94,6 +94,7 @@ __global__ __launch_bounds__((fwd_threads<W, H, NP>())) void k_fwd_xy(PhaseArgs 
     // staged planes before any spectrum row is written over them
     {
         constexpr int RP = W + 1;
+        static_assert(T >= NP * (H / 2), "one row pair per thread");
         const bool rows = tid < NP * (H / 2);
         const int p = tid / (H / 2), y = 2 * (tid % (H / 2));
         float2 line[W];
