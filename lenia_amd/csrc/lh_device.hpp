@@ -640,11 +640,14 @@ constexpr int LOCAL_MAXT = 256;
 // with it (a few registers spilled; 1,024 worlds: 32^3 797 -> 681 us, 36^3 1300 -> 1181
 // us); workgroups of three waves (24^3, 44 x 24 x 44) ran slower so (372 -> 406 us,
 // 1273 -> 1454 us) and keep what the compiler picks.  The plain kernels need fewer
-// registers: about 60 without the SRT transform (eight waves as compiled), six waves with it.
+// registers: about 60 without the SRT transform (eight waves as compiled), six waves with
+// it, also in workgroups of three waves (85 and 87 -> 80 VGPRs, none spilled; 1,024 worlds:
+// 24^3 286 -> 246 us, 44 x 24 x 44 847 -> 756 us).
 template <int W, int H, int D, int CM, bool SRTL, bool PLAIN>
 constexpr int local_sweep_waves() {
+    if (CM == 3 && PLAIN && SRTL) return 6;
     if (CM != 3 || (SweepCfg<W, H, D, CM, LOCAL_MAXT>::THREADS + 63) / 64 != 4) return 1;
-    return PLAIN ? (SRTL ? 6 : 1) : (SRTL ? 5 : 6);
+    return PLAIN ? 1 : (SRTL ? 5 : 6);
 }
 // PLAIN: every world of the launch is a plain_local_world (the optional paths compiled out).
 template <int W, int H, int D, int CM, bool SRTL, bool PLAIN>
